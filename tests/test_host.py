@@ -1,0 +1,214 @@
+"""CPU-only tests: the C-ABI library, the planner, precision rules, hook tracing and
+kernel compilation (hiprtc cross-compiles gfx950 without a GPU).  No test here
+launches device work."""
+import ctypes
+import os
+import re
+from types import SimpleNamespace
+
+import numpy as np
+import pytest
+
+import vkdispatch_amd as vd
+import vkdispatch_amd.codegen as vc
+import vkdispatch_amd.fft.functions as fft_functions
+from vkdispatch_amd import runtime as rt
+from vkdispatch_amd.fft import kernels as K
+from vkdispatch_amd.fft import planner as P
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "vkdfft.h")
+
+
+def _header_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(vkx_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_header_symbol():
+    names = _header_functions()
+    assert len(names) >= 30
+    lib = rt.lib()
+    for n in names:
+        assert hasattr(lib, n), f"libvkdfft.so does not export {n}"
+    assert set(names) == set(rt.exported_symbols())
+
+
+def test_library_basic_calls_without_device():
+    lib = rt.lib()
+    assert lib.vkx_version() == 100
+    lib.vkx_clear_error()
+    assert lib.vkx_last_error() == b""
+
+
+# ---------------------------------------------------------------------------
+# planner
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("n", [n for n in range(1, 65) if n == 1 or max(P.prime_factors(n)) <= 31] +
+                         [81, 100, 128, 162, 243, 256, 500, 512, 720, 1000, 1024, 1859, 2048, 4096, 8192, 10000,
+                          16384])
+def test_radix_plan_invariants(n):
+    radices, tt = P.radix_plan(n)
+    assert int(np.prod(radices)) == n
+    assert 1 <= tt <= 1024
+    assert all(r <= 31 or n == r for r in radices)
+    vmax = max(P.slots(n, r, tt) for r in radices)
+    assert vmax <= 36
+    # every butterfly of every pass is covered by some thread
+    for r in radices:
+        assert -(-(n // r) // tt) * tt >= n // r
+
+
+def test_plan_geometry_matches_reference_axis_rules():
+    g = P.AxisGeometry.of((64, 512, 257), 1)
+    assert (g.N, g.S, g.OUTER) == (512, 257, 64)
+    g = P.AxisGeometry.of((7, 5, 3), 0)
+    assert (g.N, g.S, g.OUTER) == (7, 15, 1)
+    g = P.AxisGeometry.of((1024, 4096))
+    assert (g.N, g.S, g.OUTER) == (4096, 1, 1024)
+
+
+def test_plan_lds_and_threads_within_gfx950_limits():
+    for shape, axis in [((1024, 4096), 1), ((64, 512, 257), 1), ((256, 256, 256), 0), ((2, 16384), 1),
+                        ((3, 4096, 7), 1), ((10, 10000), 1)]:
+        plan = P.make_plan(P.AxisGeometry.of(shape, axis))
+        assert plan.BLOCK <= 1024
+        assert plan.lds_bytes <= 160 * 1024
+        assert plan.NWG >= 1
+
+
+def test_pad_dim_and_primes_like_reference():
+    from vkdispatch_amd.fft.prime_utils import pad_dim, prime_factors, group_primes
+    assert prime_factors(360) == [2, 2, 2, 3, 3, 5]
+    assert group_primes([2, 2, 2, 3, 3, 5], 16) == [[2, 2, 2], [3, 3], [5]]
+    assert pad_dim(17) == 18 and pad_dim(1021) == 1024 and pad_dim(512) == 512
+    with pytest.raises(ValueError):
+        pad_dim(0)
+
+
+def test_large_prime_needs_bluestein():
+    with pytest.raises(ValueError):
+        P.radix_plan(2 * 37)
+
+
+# ---------------------------------------------------------------------------
+# precision rules (ports of the reference's fake-buffer tests,
+# tests/test_fft_mixed_precision.py:270-320)
+# ---------------------------------------------------------------------------
+class _FakeBuffer:
+    def __init__(self, var_type):
+        self.var_type = var_type
+
+
+def test_resolve_input_precision_output_map_infers_input_from_post_map_argument(monkeypatch):
+    monkeypatch.setattr(fft_functions.prec, "ensure_supported_complex_precision", lambda dtype, role: None)
+    output_map = SimpleNamespace(buffer_types=[vc.Buffer[vd.complex64], vc.Buffer[vd.float32]])
+    resolved = fft_functions._resolve_input_precision(
+        (_FakeBuffer(vd.complex64), _FakeBuffer(vd.float32), _FakeBuffer(vd.complex128)),
+        input_map=None, output_map=output_map, input_type=None, output_precision=None)
+    assert resolved is vd.complex128
+
+
+def test_resolve_input_precision_output_map_requires_input_buffer_after_map_args(monkeypatch):
+    monkeypatch.setattr(fft_functions.prec, "ensure_supported_complex_precision", lambda dtype, role: None)
+    output_map = SimpleNamespace(buffer_types=[vc.Buffer[vd.complex64]])
+    with pytest.raises(ValueError, match="input buffer argument must be provided"):
+        fft_functions._resolve_input_precision((_FakeBuffer(vd.complex64),), input_map=None, output_map=output_map,
+                                               input_type=None, output_precision=None)
+
+
+def test_compute_precision_defaults():
+    from vkdispatch_amd.fft import precision as prec
+    assert prec.resolve_compute_precision([vd.complex32], None) is vd.complex64
+    assert prec.resolve_compute_precision([vd.complex32, vd.complex128], None) is vd.complex128
+    assert prec.resolve_compute_precision([], None) is vd.complex64
+    assert prec.resolve_compute_precision([vd.complex64], vd.complex128) is vd.complex128
+    with pytest.raises(ValueError):
+        prec.resolve_compute_precision([vd.float32], None)
+
+
+def test_in_place_type_mismatch_rejected():
+    with pytest.raises(ValueError, match="in-place"):
+        fft_functions._resolve_input_precision((_FakeBuffer(vd.complex64),), None, None, vd.complex128, vd.complex64)
+
+
+# ---------------------------------------------------------------------------
+# hook tracing and kernel compilation
+# ---------------------------------------------------------------------------
+def test_trace_default_and_hooked_sources():
+    src = vd.fft.make_fft_shader((8, 256), None).source
+    assert "vkx::fft_body<Cfg, HookIn, HookOut>" in src and "DefaultIn<0, float>" in src
+
+    @vd.map
+    def scale(s: vc.Var[vc.f32]):
+        op = vd.fft.read_op()
+        op.register[:] = op.register * s
+
+    src = vd.fft.make_convolution_shader((8, 256), scale, 1, None).source
+    assert "vkx::cval<float>(prm, 0)" in src and "conv_body" in src
+
+
+def test_hook_argument_order_matches_reference():
+    """output-map args, then input-map args, then kernel-map args (io_manager.py:86-102)."""
+    @vd.map
+    def om(a: vc.Buffer[vd.complex64], s: vc.Var[vd.float32]):
+        vd.fft.write_op().write_to_buffer(a)
+
+    @vd.map
+    def im(b: vc.Buffer[vd.float32]):
+        op = vd.fft.read_op()
+        op.register.real = b[op.io_index]
+
+    @vd.map
+    def km(c: vc.Buffer[vd.complex64], t: vc.Const[vd.int32]):
+        pass
+
+    k = vd.fft.make_convolution_shader((4, 64), km, 1, None, input_map=im, output_map=om)
+    kinds = [(a.role, a.kind, a.dtype.name, a.slot) for a in k.args]
+    assert kinds == [("output", "buffer", "complex64", 0), ("output", "scalar", "float32", 0),
+                     ("input", "buffer", "float32", 1), ("kernel", "buffer", "complex64", 2),
+                     ("kernel", "scalar", "int32", 4)]
+
+
+def test_tracer_expressions():
+    with vc.trace_scope() as b:
+        r = vc.new_register(vd.complex64)
+        u = vc.new_uint_register(5)
+        r[:] = vc.mult_complex(r, r.conjugate())
+        r.real = r.imag * 2
+        idx = vc.ravel_index(u, (3, 4, 5))
+        flat = vc.unravel_index(idx, (3, 4, 5))
+        v = flat.to_register()
+    text = "\n".join(b.lines)
+    assert "vkx::cmul" in text and "vkx::conj" in text and ".x =" in text
+    assert v.var_type is vd.uint32
+
+
+def test_bind_packs_buffers_and_scalars():
+    @vd.map
+    def km(s: vc.Var[vd.float32], c: vc.Const[vd.complex64]):
+        pass
+
+    k = vd.fft.make_convolution_shader((4, 64), km, 1, None)
+    buf = SimpleNamespace(ptr=0x1000, size=256, var_type=vd.complex64)
+    ptrs, consts = k.bind([buf, 2.5, 1 - 2j])
+    assert ptrs == [0x1000]
+    assert np.frombuffer(consts[:4], np.float32)[0] == 2.5
+    assert np.allclose(np.frombuffer(consts[8:16], np.float32), [1, -2])
+    with pytest.raises(ValueError):
+        k.bind([buf])
+
+
+@pytest.mark.parametrize("make", [
+    lambda: vd.fft.make_fft_shader((1024, 4096), None),
+    lambda: vd.fft.make_fft_shader((64, 512, 257), 1, inverse=True),
+    lambda: vd.fft.make_fft_shader((6, 162), None, r2c=True, inverse=True),
+    lambda: vd.fft.make_fft_shader((16, 720), None, compute_type=vd.complex128, output_type=vd.complex128,
+                                   input_type=vd.complex128),
+    lambda: vd.fft.make_convolution_shader((64, 512, 257), None, 1, 1, kernel_inner_only=True),
+    lambda: vd.fft.make_transpose_shader((3, 64, 33), 1),
+])
+def test_kernels_compile_for_gfx950(make, tmp_path, monkeypatch):
+    monkeypatch.setenv("VKX_KERNEL_CACHE", str(tmp_path))
+    code = K.compile_source(make().source)
+    assert code[:4] == b"\x7fELF"

@@ -1,0 +1,272 @@
+"""Instantiation, compilation and launch of FFT-family kernels.
+
+For each plan the host writes a short HIP translation unit: a ``struct Cfg``
+of compile-time constants, the I/O functors (defaults from ``vkx_fft.h`` or
+traced mapping hooks) and an ``extern "C"`` entry that calls one of the
+hand-written bodies (``fft_body``, ``conv_body``, ``transpose_body``).  It is
+compiled for the device arch with hiprtc through ``vkx_compile`` (with packed
+FP32 disabled) and cached on disk by content hash, then loaded with
+``vkx_module_load`` and bound to its twiddle tables with ``vkx_plan_create``.
+"""
+from __future__ import annotations
+
+import ctypes
+import hashlib
+import os
+import threading
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from .. import dtypes as dt
+from .. import runtime as rt
+from .planner import KernelPlan
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DEVICE_INCLUDE = os.path.join(PKG_DIR, "csrc", "device")
+HEADER = os.path.join(DEVICE_INCLUDE, "vkx_fft.h")
+
+JIT_FLAGS = ["-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+
+def arch() -> str:
+    return os.environ.get("VKX_OFFLOAD_ARCH", "gfx950")
+
+
+def cache_dir() -> str:
+    d = os.environ.get("VKX_KERNEL_CACHE")
+    if d:
+        return d
+    return os.path.join(PKG_DIR, "_kcache")
+
+
+_header_digest = None
+
+
+def _header_hash() -> str:
+    global _header_digest
+    if _header_digest is None:
+        h = hashlib.sha256()
+        for name in sorted(os.listdir(DEVICE_INCLUDE)):
+            if name.endswith(".h"):
+                with open(os.path.join(DEVICE_INCLUDE, name), "rb") as f:
+                    h.update(name.encode())
+                    h.update(f.read())
+        _header_digest = h.hexdigest()
+    return _header_digest
+
+
+def source_key(source: str) -> str:
+    h = hashlib.sha256()
+    h.update(source.encode())
+    h.update(_header_hash().encode())
+    h.update(arch().encode())
+    h.update(" ".join(JIT_FLAGS).encode())
+    return h.hexdigest()[:40]
+
+
+def compile_source(source: str, name: str = "vkx_kernel.hip") -> bytes:
+    """Compile (or fetch from the on-disk cache) the code object of ``source``."""
+    key = source_key(source)
+    d = cache_dir()
+    path = os.path.join(d, key + ".hsaco")
+    if os.path.exists(path):
+        with open(path, "rb") as f:
+            return f.read()
+    opts = [f"--offload-arch={arch()}", *JIT_FLAGS, f"-I{DEVICE_INCLUDE}"]
+    arr = (ctypes.c_char_p * len(opts))(*[o.encode() for o in opts])
+    code = ctypes.c_void_p()
+    size = ctypes.c_size_t()
+    log = ctypes.c_void_p()
+    L = rt.lib()
+    rc = L.vkx_compile(source.encode(), name.encode(), arr, len(opts), ctypes.byref(code), ctypes.byref(size),
+                       ctypes.byref(log))
+    log_text = ""
+    if log.value:
+        log_text = ctypes.string_at(log.value).decode(errors="replace")
+        L.vkx_free_host(log)
+    if rc != 0:
+        err = L.vkx_last_error().decode(errors="replace")
+        L.vkx_clear_error()
+        raise RuntimeError(f"kernel compilation failed: {err}\n{log_text}\n--- source ---\n{source}")
+    data = ctypes.string_at(code.value, size.value)
+    L.vkx_free_host(code)
+    try:
+        os.makedirs(d, exist_ok=True)
+        tmp = f"{path}.{os.getpid()}.{threading.get_ident()}.tmp"
+        with open(tmp, "wb") as f:
+            f.write(data)
+        os.replace(tmp, path)
+    except OSError:
+        pass
+    return data
+
+
+# ---------------------------------------------------------------------------
+# source assembly
+# ---------------------------------------------------------------------------
+def _pl(passes: Sequence[Tuple[int, int]]) -> str:
+    return "vkx::PL<" + ", ".join(f"vkx::P<{r}, {ns}>" for r, ns in passes) + ">"
+
+
+def cfg_struct(plan: KernelPlan, *, in_os: int, out_os: int, k_os: int, sig: Tuple[int, int], r2c: bool,
+               inverse: bool, normalize: bool, scale: float, knum: int = 1, k_inner_only: bool = False,
+               k_transposed: bool = False) -> str:
+    b = lambda v: "true" if v else "false"  # noqa: E731
+    lines = [
+        "struct Cfg {",
+        f"  using real = {plan.real};",
+        f"  static constexpr int N = {plan.N}, TT = {plan.TT}, B = {plan.B}, BLOCK = {plan.BLOCK}, VMAX = {plan.VMAX};",
+        f"  static constexpr bool COLFAST = {b(plan.COLFAST)}, WAVE_LOCAL = {b(plan.WAVE_LOCAL)};",
+        f"  static constexpr vkx::u32 S = {plan.S}u, OUTER = {plan.OUTER}u, NWG_INNER = {plan.NWG_INNER}u;",
+        f"  static constexpr vkx::u32 IN_OS = {in_os}u, OUT_OS = {out_os}u, K_OS = {k_os}u;",
+        f"  static constexpr int PAD_SHIFT = {plan.PAD_SHIFT}, NP = {plan.NP}, LDS_ELEMS = {plan.LDS_ELEMS};",
+        f"  static constexpr int SIG_LO = {sig[0]}, SIG_HI = {sig[1]};",
+        f"  static constexpr bool R2C = {b(r2c)}, INV = {b(inverse)}, NORMALIZE = {b(normalize)};",
+        f"  static constexpr double SCALE = {scale!r};",
+        f"  static constexpr int KNUM = {knum}, KSLOTS = {plan.KSLOTS};",
+        f"  static constexpr bool KERNEL_INNER_ONLY = {b(k_inner_only)}, KERNEL_TRANSPOSED = {b(k_transposed)};",
+        f"  using fwd = {_pl(plan.fwd)};",
+        f"  using inv = {_pl(plan.inv)};",
+        "};",
+    ]
+    return "\n".join(lines)
+
+
+def translation_unit(cfg: str, functors: Sequence[str], body: str, block: int) -> str:
+    parts = ['#include "vkx_fft.h"', "", cfg, ""]
+    parts += list(functors)
+    parts += ["", f'extern "C" __global__ void __launch_bounds__({block}) vkx_main(vkx::KParams prm) {{',
+              f"  {body}", "}", ""]
+    return "\n".join(parts)
+
+
+# ---------------------------------------------------------------------------
+# loaded kernels
+# ---------------------------------------------------------------------------
+_modules: Dict[str, Tuple[int, int]] = {}
+_mod_lock = threading.Lock()
+
+
+def load_function(source: str) -> int:
+    key = source_key(source)
+    with _mod_lock:
+        if key in _modules:
+            return _modules[key][1]
+    code = compile_source(source)
+    rt.ensure_device()
+    L = rt.lib()
+    mod = ctypes.c_uint64(0)
+    buf = ctypes.create_string_buffer(code, len(code))
+    rt.check(L.vkx_module_load(ctypes.byref(mod), buf, len(code)), "vkx_module_load")
+    fn = ctypes.c_uint64(0)
+    rt.check(L.vkx_module_function(ctypes.byref(fn), mod.value, b"vkx_main"), "vkx_module_function")
+    with _mod_lock:
+        _modules[key] = (mod.value, fn.value)
+    return fn.value
+
+
+class CompiledKernel:
+    """One FFT / convolution / transpose dispatch: source + plan + argument layout."""
+
+    def __init__(self, source: str, plan: KernelPlan, args, twiddle_fwd: Optional[List[int]],
+                 twiddle_inv: Optional[List[int]], name: str = "vkx_fft"):
+        self.source = source
+        self.plan = plan
+        self.args = list(args)
+        self.name = name
+        self.tw_fwd = twiddle_fwd
+        self.tw_inv = twiddle_inv
+        self._plan_handle = None
+        self._lock = threading.Lock()
+
+    def __str__(self):
+        return self.source
+
+    @property
+    def num_buffers(self) -> int:
+        return sum(1 for a in self.args if a.kind == "buffer")
+
+    def handle(self) -> int:
+        if self._plan_handle is None:
+            with self._lock:
+                if self._plan_handle is None:
+                    fn = load_function(self.source)
+                    d = rt.PlanDesc()
+                    d.function = fn
+                    for i, v in enumerate(self.plan.grid):
+                        d.grid[i] = v
+                    d.block[0], d.block[1], d.block[2] = self.plan.BLOCK, 1, 1
+                    d.shared_bytes = 0
+                    d.n = self.plan.N
+                    d.double_precision = 1 if self.plan.real == "double" else 0
+                    if self.tw_fwd:
+                        d.num_passes_fwd = len(self.tw_fwd)
+                        for i, r in enumerate(self.tw_fwd):
+                            d.radices_fwd[i] = r
+                    if self.tw_inv:
+                        d.num_passes_inv = len(self.tw_inv)
+                        for i, r in enumerate(self.tw_inv):
+                            d.radices_inv[i] = r
+                    h = ctypes.c_uint64(0)
+                    rt.check(rt.lib().vkx_plan_create(ctypes.byref(h), ctypes.byref(d)), "vkx_plan_create")
+                    self._plan_handle = h.value
+        return self._plan_handle
+
+    def bind(self, values: Sequence) -> Tuple[List[int], bytes]:
+        """Device pointers and packed constants for positional ``values``."""
+        if len(values) != len(self.args):
+            raise ValueError(f"{self.name} expects {len(self.args)} arguments "
+                             f"({', '.join(a.kind + ':' + a.dtype.name for a in self.args)}), got {len(values)}")
+        ptrs: List[int] = []
+        consts = bytearray(256)
+        used = 0
+        for a, v in zip(self.args, values):
+            if a.kind == "buffer":
+                ptr = getattr(v, "ptr", None)
+                if ptr is None:
+                    raise ValueError(f"Expected a buffer for {a.role} argument, got {v!r}")
+                ptrs.append(int(ptr))
+            else:
+                if hasattr(v, "ptr"):
+                    raise ValueError(f"Expected a scalar value for {a.role} argument, got a buffer")
+                if a.dtype.is_complex:
+                    c = complex(v)
+                    raw = np.array([c.real, c.imag], dtype=dt.storage_scalar_numpy(a.dtype)).tobytes()
+                elif a.dtype is dt.bfloat16:
+                    raw = dt.bf16_bits_from_float32(np.array([v], np.float32)).tobytes()
+                else:
+                    raw = np.array([v], dtype=a.dtype.numpy).tobytes()
+                consts[a.slot:a.slot + len(raw)] = raw
+                used = max(used, a.slot + len(raw))
+        return ptrs, bytes(consts[:max(used, 0)])
+
+    def launch(self, values: Sequence, stream: Optional[int] = None) -> None:
+        ptrs, consts = self.bind(values)
+        self.launch_raw(ptrs, consts, stream)
+
+    def launch_raw(self, ptrs: Sequence[int], consts: bytes = b"", stream: Optional[int] = None) -> None:
+        h = self.handle()
+        arr = (ctypes.c_uint64 * max(1, len(ptrs)))(*ptrs)
+        cbuf = ctypes.create_string_buffer(consts, max(1, len(consts)))
+        s = rt.current_stream() if stream is None else int(stream)
+        rt.check(rt.lib().vkx_plan_exec(h, arr, len(ptrs), cbuf, len(consts), s), f"{self.name} launch")
+
+
+def precompile(kernels, workers: int = 0) -> int:
+    """Compile the code objects of ``kernels`` (CompiledKernel or source strings) in
+    parallel into the on-disk cache; returns how many were compiled."""
+    from concurrent.futures import ThreadPoolExecutor
+    srcs = []
+    for k in kernels:
+        s = k if isinstance(k, str) else k.source
+        if not os.path.exists(os.path.join(cache_dir(), source_key(s) + ".hsaco")):
+            srcs.append(s)
+    srcs = list(dict.fromkeys(srcs))
+    if not srcs:
+        return 0
+    if workers <= 0:
+        workers = min(16, os.cpu_count() or 4)
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        list(ex.map(compile_source, srcs))
+    return len(srcs)

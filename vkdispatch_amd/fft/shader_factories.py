@@ -1,0 +1,176 @@
+"""Kernel factories of vd.fft (counterparts of vkdispatch/fft/shader_factories.py).
+
+``make_fft_shader`` / ``make_convolution_shader`` / ``make_transpose_shader``
+are ``lru_cache``d on all their arguments, like the reference's
+(shader_factories.py:11, 72, 106); mapping functions hash by instance.  Each
+returns a :class:`CompiledKernel` whose source is the HIP instantiation.
+"""
+from __future__ import annotations
+
+from functools import lru_cache
+from typing import Optional, Tuple
+
+from .. import dtypes as dt
+from . import hooks
+from .kernels import CompiledKernel, cfg_struct, translation_unit
+from .planner import AxisGeometry, KernelPlan, make_plan, transposed_size
+from .precision import kernel_real_type
+
+
+def _compute_dtype(compute_type):
+    return dt.complex128 if compute_type is dt.complex128 else dt.complex64
+
+
+def _resolve_signal_range(signal_range, N: int) -> Tuple[int, int]:
+    if signal_range is None:
+        return 0, N
+    s, e = signal_range
+    s = 0 if s is None else int(s)
+    e = N if e is None else int(e)
+    if s < 0 or e > N or s > e:
+        raise ValueError(f"input_signal_range {signal_range} is not within [0, {N}]")
+    return s, e
+
+
+def conv_plan(buffer_shape, axis, compute_type) -> KernelPlan:
+    geom = AxisGeometry.of(buffer_shape, axis)
+    return make_plan(geom, kernel_real_type(compute_type))
+
+
+def _io_functors(args, output_map, input_map, output_type, input_type, compute, N, knum):
+    out_args = [a for a in args if a.role == "output"]
+    in_args = [a for a in args if a.role == "input"]
+    functors = []
+    if output_map is not None:
+        functors.append(hooks.functor_source("HookOut", output_map, out_args, compute, N, writing=True,
+                                             per_kernel=True, kernel_num=knum))
+    else:
+        functors.append(f"using HookOut = vkx::DefaultOut<{out_args[0].slot}, {output_type.child_type.cpp}>;")
+    if input_map is not None:
+        functors.append(hooks.functor_source("HookIn", input_map, in_args, compute, N, writing=False,
+                                             per_kernel=False))
+    elif output_map is not None:
+        functors.append(f"using HookIn = vkx::DefaultIn<{in_args[0].slot}, {input_type.child_type.cpp}>;")
+    else:
+        functors.append(f"using HookIn = vkx::DefaultIn<{out_args[0].slot}, {input_type.child_type.cpp}>;")
+    return functors
+
+
+@lru_cache(maxsize=None)
+def make_fft_shader(buffer_shape: Tuple, axis: int = None, inverse: bool = False, normalize_inverse: bool = True,
+                    r2c: bool = False, input_map=None, output_map=None, input_type=None, output_type=None,
+                    compute_type=None, input_signal_range: Optional[Tuple] = None) -> CompiledKernel:
+    if output_type is None:
+        output_type = dt.complex64
+    if input_type is None and input_map is None:
+        input_type = output_type
+    if compute_type is None:
+        compute_type = dt.complex64
+    compute = _compute_dtype(compute_type)
+    geom = AxisGeometry.of(buffer_shape, axis)
+    N = geom.N
+    if r2c and geom.S != 1:
+        raise ValueError("R2C io only supported for contiguous data")
+    plan = make_plan(geom, kernel_real_type(compute_type))
+    args = hooks.declare_args(output_map, input_map, None, output_type, input_type, None, conv=False)
+    if r2c:
+        half = N // 2 + 1
+        in_os, out_os = (half, 2 * half) if inverse else (2 * half, half)
+    else:
+        in_os = out_os = N * geom.S
+    sig = _resolve_signal_range(input_signal_range, N)
+    normalize = bool(inverse and normalize_inverse)
+    cfg = cfg_struct(plan, in_os=in_os, out_os=out_os, k_os=N * geom.S, sig=sig, r2c=r2c, inverse=inverse,
+                     normalize=normalize, scale=1.0 / N)
+    functors = _io_functors(args, output_map, input_map, output_type, input_type, compute, N, 1)
+    src = translation_unit(cfg, functors, "vkx::fft_body<Cfg, HookIn, HookOut>(prm);", plan.BLOCK)
+    name = f"fft_shader_{tuple(buffer_shape)}_{axis}_{inverse}_{normalize_inverse}_{r2c}"
+    return CompiledKernel(src, plan, args, list(plan.radices), None, name=name)
+
+
+@lru_cache(maxsize=None)
+def get_transposed_size(buffer_shape: Tuple, axis: int = None, compute_type=dt.complex64) -> int:
+    return transposed_size(conv_plan(tuple(buffer_shape), axis, compute_type))
+
+
+@lru_cache(maxsize=None)
+def make_transpose_shader(buffer_shape: Tuple, axis: int = None, kernel_inner_only: bool = False,
+                          input_type=dt.complex64, output_type=dt.complex64,
+                          compute_type=dt.complex64) -> CompiledKernel:
+    geom = AxisGeometry.of(buffer_shape, axis)
+    plan = conv_plan(tuple(buffer_shape), axis, compute_type)
+    from .hooks import Arg
+    args = [Arg("output", "buffer", output_type, 0), Arg("input", "buffer", input_type, 1)]
+    cfg = cfg_struct(plan, in_os=geom.N * geom.S, out_os=geom.N * geom.S, k_os=geom.N * geom.S, sig=(0, geom.N),
+                     r2c=False, inverse=False, normalize=False, scale=1.0, k_inner_only=kernel_inner_only,
+                     k_transposed=True)
+    body = (f"vkx::transpose_body<Cfg, {input_type.child_type.cpp}, {output_type.child_type.cpp}>(prm);")
+    src = translation_unit(cfg, [], body, plan.BLOCK)
+    return CompiledKernel(src, plan, args, None, None, name=f"transpose_shader_{tuple(buffer_shape)}_{axis}")
+
+
+_kernel_index_doc = "index of the kernel being traced in a kernel_num > 1 convolution (None otherwise)"
+
+
+def mapped_kernel_index():
+    return hooks.mapped_kernel_index()
+
+
+@lru_cache(maxsize=None)
+def make_convolution_shader(buffer_shape: Tuple, kernel_map=None, kernel_num: int = 1, axis: int = None,
+                            normalize: bool = True, transposed_kernel: bool = False,
+                            kernel_inner_only: bool = False, input_map=None, output_map=None, input_type=None,
+                            output_type=None, kernel_type=None, compute_type=None,
+                            input_signal_range: Optional[Tuple] = None) -> CompiledKernel:
+    if output_type is None:
+        output_type = dt.complex64
+    if input_type is None and input_map is None:
+        input_type = output_type
+    if kernel_type is None:
+        kernel_type = dt.complex64
+    if compute_type is None:
+        compute_type = dt.complex64
+    if kernel_num < 1:
+        raise ValueError("kernel_num must be >= 1")
+    compute = _compute_dtype(compute_type)
+    geom = AxisGeometry.of(buffer_shape, axis)
+    N = geom.N
+    plan = conv_plan(tuple(buffer_shape), axis, compute_type)
+    args = hooks.declare_args(output_map, input_map, kernel_map, output_type, input_type, kernel_type, conv=True)
+    k_args = [a for a in args if a.role == "kernel"]
+    sig = _resolve_signal_range(input_signal_range, N)
+    cfg = cfg_struct(plan, in_os=N * geom.S, out_os=N * geom.S, k_os=N * geom.S, sig=sig, r2c=False,
+                     inverse=False, normalize=bool(normalize), scale=1.0 / N, knum=kernel_num,
+                     k_inner_only=kernel_inner_only, k_transposed=transposed_kernel)
+    functors = _io_functors(args, output_map, input_map, output_type, input_type, compute, N, kernel_num)
+    if kernel_map is not None:
+        functors.append(hooks.functor_source("HookKernel", kernel_map, k_args, compute, N, writing=False,
+                                             per_kernel=True, kernel_num=kernel_num))
+    else:
+        functors.append(f"using HookKernel = vkx::DefaultKernel<{k_args[0].slot}, {kernel_type.child_type.cpp}>;")
+    src = translation_unit(cfg, functors, "vkx::conv_body<Cfg, HookIn, HookKernel, HookOut>(prm);", plan.BLOCK)
+    return CompiledKernel(src, plan, args, list(plan.radices), list(reversed(plan.radices)),
+                          name=f"convolution_shader_{tuple(buffer_shape)}_{axis}")
+
+
+def get_cache_info():
+    return make_fft_shader.cache_info()
+
+
+def get_convoliution_cache_info():  # reference spelling
+    return make_convolution_shader.cache_info()
+
+
+get_convolution_cache_info = get_convoliution_cache_info
+
+
+def print_cache_info():
+    print(get_cache_info())
+    print(get_convoliution_cache_info())
+
+
+def cache_clear():
+    make_convolution_shader.cache_clear()
+    make_fft_shader.cache_clear()
+    make_transpose_shader.cache_clear()
+    get_transposed_size.cache_clear()

@@ -1,0 +1,116 @@
+"""``vd.CommandGraph`` on HIP graphs.
+
+The reference records dispatches into a CommandGraph and submits them through
+its native queues (vkdispatch/execution_pipeline/command_graph.py:38-319); the
+default thread-local graph submits on record (:321-348).  Here a graph records
+bound launches (kernel plan, device pointers, packed constants); ``submit``
+captures them once into a hipGraph on the current stream (plans and twiddle
+tables are created at record time, so capture never allocates) and replays the
+instantiated graph on every later submit.
+"""
+from __future__ import annotations
+
+import ctypes
+import threading
+from typing import List, Optional, Tuple
+
+from . import runtime as rt
+
+
+class CommandGraph:
+    def __init__(self, reset_on_submit: bool = False, submit_on_record: bool = False):
+        self.reset_on_submit = reset_on_submit
+        self.submit_on_record = submit_on_record
+        self._records: List[Tuple[object, List[int], bytes]] = []
+        self._exec: Optional[int] = None
+        self._exec_stream: Optional[int] = None
+
+    def record_kernel(self, kernel, values) -> None:
+        ptrs, consts = kernel.bind(values)
+        kernel.handle()
+        if self.submit_on_record:
+            kernel.launch_raw(ptrs, consts)
+            return
+        self._records.append((kernel, ptrs, consts))
+        self._drop_exec()
+
+    def record_callable(self, fn) -> None:
+        """Record a host-side launcher ``fn(stream)`` (used by fused multi-kernel paths)."""
+        if self.submit_on_record:
+            fn(rt.current_stream())
+            return
+        self._records.append((fn, None, None))
+        self._drop_exec()
+
+    def __len__(self):
+        return len(self._records)
+
+    def _drop_exec(self):
+        if self._exec is not None:
+            rt.lib().vkx_graph_destroy(self._exec)
+            self._exec = None
+
+    def _replay(self, stream: int) -> None:
+        for k, ptrs, consts in self._records:
+            if ptrs is None:
+                k(stream)
+            else:
+                k.launch_raw(ptrs, consts, stream)
+
+    def capture(self, stream: Optional[int] = None) -> int:
+        s = rt.current_stream() if stream is None else int(stream)
+        if self._exec is None or self._exec_stream != s:
+            self._drop_exec()
+            rt.check(rt.lib().vkx_graph_begin(s), "vkx_graph_begin")
+            try:
+                self._replay(s)
+            finally:
+                h = ctypes.c_uint64(0)
+                rt.check(rt.lib().vkx_graph_end(s, ctypes.byref(h)), "vkx_graph_end")
+            self._exec, self._exec_stream = h.value, s
+        return self._exec
+
+    def submit(self, instance_count: Optional[int] = None, queue_index: int = -2, stream=None,
+               use_graph: bool = True) -> None:
+        if not self._records:
+            return
+        s = rt.current_stream() if stream is None else int(stream)
+        reps = 1 if instance_count is None else int(instance_count)
+        if use_graph:
+            ex = self.capture(s)
+            for _ in range(reps):
+                rt.check(rt.lib().vkx_graph_launch(ex, s), "vkx_graph_launch")
+        else:
+            for _ in range(reps):
+                self._replay(s)
+        if self.reset_on_submit:
+            self.reset()
+
+    def reset(self) -> None:
+        self._drop_exec()
+        self._records = []
+
+    def destroy(self) -> None:
+        self.reset()
+
+    def __del__(self):
+        try:
+            self._drop_exec()
+        except Exception:
+            pass
+
+
+_tls = threading.local()
+
+
+def global_graph() -> CommandGraph:
+    g = getattr(_tls, "graph", None)
+    if g is None:
+        g = _tls.graph = CommandGraph(reset_on_submit=True, submit_on_record=True)
+    return g
+
+
+def dispatch(kernel, values, graph: Optional[CommandGraph] = None) -> None:
+    if graph is not None and not isinstance(graph, CommandGraph):
+        raise ValueError("Expected a CommandGraph object for 'graph'!")
+    (graph if graph is not None else global_graph()).record_kernel(kernel, values)
