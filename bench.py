@@ -1,0 +1,223 @@
+"""Benchmark of the vd.fft FFT-convolution path on one MI355X.
+
+Default workload (the metric's configuration): FP32 2-D FFT-convolution,
+64 real float32 images of 512x512 drawn U(-1, 1) from seed 0, one 512x512
+kernel = isotropic Gaussian (sigma 4 px, circular, peak 1 at (0,0)) + 1%
+N(0,1) noise, pre-transformed K = rfft2(k) (not timed), broadcast over the 64
+images.  One step = ``vd.fft.convolve2DR(images, K, kernel_inner_only=True)``:
+irfft2(rfft2(x) * conj(K)) in place.  Every step runs on its own resident copy
+of the input (all copies in HBM before timing), so repeated in-place steps do
+not feed results back into inputs.
+
+Output: one JSON line with the metric (GB/s of algorithmic bytes: 64 MiB in,
+64 MiB out, 1 MiB kernel spectrum per step), the roofline of the dominant
+kernel (HIP events on the launch stream), the FP64-oracle parity of the
+result, and the CPU baseline (the FP64 oracle, single core, on a bounded
+sample; plus multithreaded pocketfft via scipy).
+
+Other workloads (``--workload c2c4096``) print their own line for reference.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def conv_inputs(batch=64, n=512, seed=0):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(-1, 1, (batch, n, n)).astype(np.float32)
+    yy, xx = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    dy, dx = np.minimum(yy, n - yy), np.minimum(xx, n - xx)
+    g = np.exp(-(dy.astype(np.float64) ** 2 + dx ** 2) / (2 * 4.0 ** 2))
+    k = (g + 0.01 * rng.standard_normal((n, n))).astype(np.float32)
+    return x, k
+
+
+def _cpu_info():
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def cpu_baseline_conv(x, K, budget_s=12.0):
+    """FP64 oracle (single core) on a bounded sample of images, scaled to GB/s."""
+    import oracle as O
+    n = x.shape[-1]
+    per_img_bytes = 2 * n * n * 4
+    imgs, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < budget_s and imgs < x.shape[0]:
+        O.convolve2DR(x[imgs:imgs + 1], K, kernel_inner_only=True)
+        imgs += 1
+    dt = time.perf_counter() - t0
+    oracle_gbs = imgs * per_img_bytes / dt / 1e9
+    extra = None
+    try:
+        import scipy.fft as sf
+        w = os.cpu_count() or 1
+        Kc = K.astype(np.complex64)
+        sf.irfft2(sf.rfft2(x[:4], workers=w) * np.conj(Kc), s=(n, n), workers=w)
+        reps, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < 5.0 and reps < 20:
+            sf.irfft2(sf.rfft2(x, workers=w) * np.conj(Kc), s=(n, n), workers=w)
+            reps += 1
+        dt2 = (time.perf_counter() - t0) / reps
+        extra = {"value": (x.nbytes * 2 + Kc.nbytes) / dt2 / 1e9, "unit": "GB/s", "cores": w,
+                 "kind": "pocketfft (scipy.fft, float32)", "sample": f"{x.shape[0]} images x {reps} reps"}
+    except Exception as e:  # scipy missing: report only the oracle
+        extra = {"error": str(e)}
+    return {"value": oracle_gbs, "unit": "GB/s", "cores": 1, "kind": "port",
+            "sample": f"{imgs} of {x.shape[0]} images through the FP64 oracle in {dt:.1f} s"}, extra
+
+
+def bench_conv(steps, warmup, args):
+    import vkdispatch_amd as vd
+    from vkdispatch_amd.fft import fused
+    import oracle as O
+
+    batch, n = 64, 512
+    x, k = conv_inputs(batch, n, 0)
+    K = np.fft.rfft2(k.astype(np.float64)).astype(np.complex64)
+    kbuf = vd.RFFTBuffer((n, n))
+    kbuf.write_fourier(K)
+    src = vd.asrfftbuffer(x)
+    bufs = []
+    for _ in range(warmup + steps):
+        b = vd.RFFTBuffer((batch, n, n))
+        vd.runtime.dtod(b.ptr, src.ptr, src.mem_size)
+        bufs.append(b)
+    vd.synchronize()
+
+    graphs = []
+    for b in bufs:
+        g = vd.CommandGraph()
+        vd.fft.convolve2DR(b, kbuf, kernel_inner_only=True, graph=g)
+        graphs.append(g)
+    for g in graphs:
+        g.capture()
+
+    for i in range(warmup):
+        graphs[i].submit()
+    vd.synchronize()
+    t0 = time.perf_counter()
+    for i in range(warmup, warmup + steps):
+        graphs[i].submit()
+    vd.synchronize()
+    wall = time.perf_counter() - t0
+    ms = wall / steps * 1e3
+
+    alg_bytes = batch * n * n * 4 * 2 + n * (n // 2 + 1) * 8
+    value = alg_bytes / (wall / steps) / 1e9
+
+    # parity on two images of the last step (same input as every step)
+    res = bufs[-1].read_real(0)
+    ref = O.convolve2DR(x[:2], K, kernel_inner_only=True)
+    rel = O.rel_l2(res[:2], ref)
+
+    # dominant-kernel timing with HIP events on the launch stream
+    prof = fused.profile_convolve2DR(bufs[0], kbuf, reps=20)
+    dom = max(prof, key=lambda p: p["ms"])
+    achieved = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom["name"],
+                "kernel_ms": round(dom["ms"], 5), "kernel_bytes": dom["bytes"]}
+    out = {
+        "metric": "FFT-conv GB/s & %HBM roofline; BF16-MFMA vs FP32 speedup at rel-L2<=1e-3",
+        "value": round(value, 2), "unit": "GB/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": round(ms, 5), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "fp32 2-D FFT-convolution, 64 x 512x512 real images, one 512x512 kernel "
+                               "(Gaussian sigma=4 + 1% noise) broadcast, convolve2DR in place",
+                   "batch": batch, "shape": [n, n], "algorithmic_bytes_per_step": alg_bytes},
+        "roofline": roofline,
+        "hbm_fraction_of_step": round(value / HBM_PEAK_GBS, 4),
+        "parity_rel_l2_vs_fp64": rel,
+        "kernels": prof,
+    }
+    if not args.no_cpu:
+        base, extra = cpu_baseline_conv(x, K)
+        out["cpu_baseline"] = base
+        out["cpu_baseline_pocketfft"] = extra
+        model, ncpu = _cpu_info()
+        out["cpu_host"] = {"model": model, "os_cpu_count": ncpu}
+    return out
+
+
+def bench_c2c(steps, warmup, args):
+    import vkdispatch_amd as vd
+    import oracle as O
+    rng = np.random.default_rng(0)
+    x = (rng.standard_normal((1024, 4096)) + 1j * rng.standard_normal((1024, 4096))).astype(np.complex64)
+    buf = vd.asbuffer(x)
+    g = vd.CommandGraph()
+    vd.fft.fft(buf, graph=g)
+    vd.fft.ifft(buf, graph=g)
+    g.capture()
+    for _ in range(warmup):
+        g.submit()
+    vd.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.submit()
+    vd.synchronize()
+    wall = (time.perf_counter() - t0) / steps
+    alg = 2 * 2 * x.nbytes
+    res = buf.read(0)
+    k_fwd = vd.fft.make_fft_shader((1024, 4096), None)
+    ms_fwd = _time_kernel(k_fwd, [buf], 20)
+    return {"metric": "C2C N=4096 x 1024 fwd+inv GB/s", "value": round(alg / wall / 1e9, 2), "unit": "GB/s",
+            "ms_per_step": round(wall * 1e3, 5), "steps": steps, "warmup": warmup,
+            "fwd_kernel_ms": ms_fwd, "fwd_kernel_gbs": round(2 * x.nbytes / (ms_fwd * 1e-3) / 1e9, 1),
+            "roundtrip_rel_l2": O.rel_l2(res, x)}
+
+
+def _time_kernel(kernel, values, reps):
+    import vkdispatch_amd as vd
+    e0, e1 = vd.Event(), vd.Event()
+    kernel.launch(values)
+    e0.record()
+    for _ in range(reps):
+        kernel.launch(values)
+    e1.record()
+    e1.synchronize()
+    return e0.elapsed_ms(e1) / reps
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="conv2d", choices=["conv2d", "c2c4096"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "vkdispatch_amd benchmarks one GPU per process; multi-GPU sharding is "
+                                   "described in DESIGN.md and not run", "n_gpus": args.gpus}))
+        sys.exit(2)
+    if args.workload == "conv2d":
+        out = bench_conv(args.steps, args.warmup, args)
+    else:
+        out = bench_c2c(args.steps, args.warmup, args)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -125,12 +125,13 @@ typedef struct vkx_plan_desc {
   int radices_fwd[VKX_MAX_PASSES];
   int num_passes_inv;             /* radix list of the inverse pass sequence (0: none) */
   int radices_inv[VKX_MAX_PASSES];
+  int roots_n;                    /* >0: bind a table exp(-2 pi i k / roots_n), k < roots_n */
 } vkx_plan_desc;
 int vkx_plan_create(uint64_t* plan, const vkx_plan_desc* desc);
 int vkx_plan_destroy(uint64_t plan);
 /* replaces descriptor_set_write_buffer + stage_compute_record + command_list_submit
  * for one FFT dispatch (vkdispatch/shader/shader_function.py:277-380).
- * bufs: device pointers in the reference's declared argument order
+ * bufs: up to 12 device pointers in the reference's declared argument order
  * (vkdispatch/fft/io_manager.py:86-102); consts: packed Var/Const values. */
 int vkx_plan_exec(uint64_t plan, const uint64_t* bufs, int num_bufs, const void* consts, size_t consts_size,
                   uint64_t stream);
@@ -139,6 +140,8 @@ int vkx_plan_exec(uint64_t plan, const uint64_t* bufs, int num_bufs, const void*
 /* Stockham twiddle table for a pass list: entry (NS_p - 1) + (r-1)*NS_p + k holds
  * exp(-2 pi i r k / (NS_p R_p)); N-1 entries in total. */
 int vkx_twiddles(uint64_t dst, int n, const int* radices, int num_passes, int double_precision, uint64_t stream);
+/* Roots of unity r[k] = exp(-2 pi i k / n), k in [0, n) (real-FFT pre/post-processing). */
+int vkx_roots(uint64_t dst, int n, int double_precision, uint64_t stream);
 /* Bluestein chirp w[m] = exp(sign * i pi m^2 / n), m in [0, count); m^2 mod 2n in exact integers. */
 int vkx_chirp(uint64_t dst, int n, int count, int sign, int double_precision, uint64_t stream);
 

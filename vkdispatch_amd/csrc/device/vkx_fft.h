@@ -275,6 +275,7 @@ struct KParams {
   u64 p[16];
   alignas(16) unsigned char c[256];
 };
+constexpr int SLOT_ROOTS = 13;   // exp(-2 pi i k / roots_n), bound by the plan
 constexpr int SLOT_TW_FWD = 14;
 constexpr int SLOT_TW_INV = 15;
 
@@ -652,6 +653,118 @@ __device__ __forceinline__ void conv_body(const KParams& prm) {
     }
     if constexpr (kidx + 1 < C::KNUM) block_sync<C>();
   });
+}
+
+// ---------------------------------------------------------------------------
+// Real-to-complex / complex-to-real rows of even length 2M through an M-point
+// complex FFT (the "half-length" method).  Same results as the reference's
+// full-length complex r2c/c2r (global_memory_iterators.py:85-92,214-227) at
+// half the arithmetic.  C::N is M; rows are contiguous; IN_OS / OUT_OS are the
+// RFFTBuffer row pitches (2M+2 reals, M+1 complex).  SLOT_ROOTS binds
+// exp(-2 pi i k / 2M).
+//
+// forward: z[m] = x[2m] + i x[2m+1]  ->  Z = FFT_M(z)
+//          E = (Z[k] + conj Z[M-k]) / 2,  O = -i (Z[k] - conj Z[M-k]) / 2
+//          X[k] = E + W^k O,  X[M-k] = conj(E - W^k O)        (k = 0..M/2)
+// inverse: Z'[k] = (X[k] + conj X[M-k]) + i conj(W^k) (X[k] - conj X[M-k])
+//          z = IFFT_M(Z'),  x[2m] = Re z[m] / 2M,  x[2m+1] = Im z[m] / 2M
+//          (imaginary parts of X[0] and X[M] do not contribute, as in a
+//           complex inverse followed by taking the real part)
+// ---------------------------------------------------------------------------
+template <class C, class S>
+__device__ __forceinline__ void rfft_body(const KParams& prm) {
+  using real = typename C::real;
+  using L = typename C::fwd;
+  using F0 = typename pl_first<L>::type;
+  using FL = typename pl_last<L>::type;
+  constexpr int M = C::N;
+  __shared__ cpx<real> lds[C::LDS_ELEMS];
+  const Geo g = make_geo<C>();
+  cpx<real> v[C::VMAX];
+  const cpx<real>* tw = bufp<const cpx<real>>(prm, SLOT_TW_FWD);
+  const cpx<real>* roots = bufp<const cpx<real>>(prm, SLOT_ROOTS);
+  const S* xin = bufp<const S>(prm, 0);
+  cpx<S>* xout = bufp<cpx<S>>(prm, 0);
+
+  if (g.valid) {
+    const u32 base = g.outer * C::IN_OS;
+    for_layout<C, F0::R>(g.t, [&](auto slot, u32 n) {
+      const cpx<S> p = *(const cpx<S>*)(xin + base + 2u * n);
+      real a = sc<real>(p.x), b = sc<real>(p.y);
+      if constexpr (C::SIG_LO > 0 || C::SIG_HI < 2 * M) {
+        const u32 e0 = 2u * n, e1 = 2u * n + 1u;
+        if (!(e0 >= (u32)C::SIG_LO && e0 < (u32)C::SIG_HI)) a = (real)0;
+        if (!(e1 >= (u32)C::SIG_LO && e1 < (u32)C::SIG_HI)) b = (real)0;
+      }
+      v[decltype(slot)::value] = cpx<real>{a, b};
+    });
+  } else {
+    sfor<0, C::VMAX>([&](auto s) { v[decltype(s)::value] = cpx<real>{(real)0, (real)0}; });
+  }
+
+  run_passes<C, -1>(L{}, v, lds, g.t, g.b, tw);
+
+  // natural-order spectrum into LDS, then Hermitian post-processing in pairs (k, M-k)
+  for_layout<C, FL::R>(g.t, [&](auto slot, u32 n) { lds[lds_addr<C>(g.b, (int)n)] = v[decltype(slot)::value]; });
+  block_sync<C>();
+  if (g.valid) {
+    const u32 ob = g.outer * C::OUT_OS;
+    const real h = (real)0.5;
+    for (int k = g.t; k <= M / 2; k += C::TT) {
+      const cpx<real> zk = lds[lds_addr<C>(g.b, k)];
+      const cpx<real> zm = lds[lds_addr<C>(g.b, k == 0 ? 0 : M - k)];
+      const cpx<real> e{h * (zk.x + zm.x), h * (zk.y - zm.y)};
+      const cpx<real> d{zk.x - zm.x, zk.y + zm.y};
+      const cpx<real> o{h * d.y, -h * d.x};
+      const cpx<real> t = cmul(roots[k], o);
+      xout[ob + (u32)k] = ccvt<S>(e + t);
+      xout[ob + (u32)(M - k)] = ccvt<S>(conj(e - t));
+    }
+  }
+}
+
+template <class C, class S>
+__device__ __forceinline__ void irfft_body(const KParams& prm) {
+  using real = typename C::real;
+  using L = typename C::fwd;
+  using F0 = typename pl_first<L>::type;
+  using FL = typename pl_last<L>::type;
+  constexpr int M = C::N;
+  __shared__ cpx<real> lds[C::LDS_ELEMS];
+  const Geo g = make_geo<C>();
+  cpx<real> v[C::VMAX];
+  const cpx<real>* tw = bufp<const cpx<real>>(prm, SLOT_TW_FWD);
+  const cpx<real>* roots = bufp<const cpx<real>>(prm, SLOT_ROOTS);
+  const cpx<S>* Xin = bufp<const cpx<S>>(prm, 0);
+  S* xout = bufp<S>(prm, 0);
+
+  if (g.valid) {
+    const u32 ib = g.outer * C::IN_OS;
+    for_layout<C, F0::R>(g.t, [&](auto slot, u32 n) {
+      cpx<real> a = ccvt<real>(Xin[ib + n]);
+      cpx<real> bm = ccvt<real>(Xin[ib + (u32)M - n]);
+      if (n == 0u) { a.y = (real)0; bm.y = (real)0; }
+      const cpx<real> e{a.x + bm.x, a.y - bm.y};
+      const cpx<real> d{a.x - bm.x, a.y + bm.y};
+      const cpx<real> o = cmul(d, conj(roots[n]));
+      v[decltype(slot)::value] = e + mul_i(o);
+    });
+  } else {
+    sfor<0, C::VMAX>([&](auto s) { v[decltype(s)::value] = cpx<real>{(real)0, (real)0}; });
+  }
+
+  run_passes<C, 1>(L{}, v, lds, g.t, g.b, tw);
+  block_sync<C>();  // every load of the row precedes every (in-place) store
+
+  if (g.valid) {
+    const u32 ob = g.outer * C::OUT_OS;
+    const real scale = (real)C::SCALE;
+    for_layout<C, FL::R>(g.t, [&](auto slot, u32 n) {
+      cpx<real> z = v[decltype(slot)::value];
+      if constexpr (C::NORMALIZE) z = z * scale;
+      *(cpx<S>*)(xout + ob + 2u * n) = ccvt<S>(z);
+    });
+  }
 }
 
 // ---------------------------------------------------------------------------

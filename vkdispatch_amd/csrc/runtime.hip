@@ -46,6 +46,8 @@ struct alignas(16) KParams {
   uint64_t p[kParamPtrs];
   unsigned char c[kParamBytes];
 };
+constexpr int kMaxUserBufs = 12;
+constexpr int kSlotRoots = 13;
 constexpr int kSlotTwFwd = 14;
 constexpr int kSlotTwInv = 15;
 
@@ -78,6 +80,17 @@ __global__ void twiddle_kernel(T* out, int n, PassInfo info) {
     }
     ns = span;
   }
+}
+
+// roots[k] = exp(-2 pi i k / n), k in [0, n)
+template <typename T>
+__global__ void roots_kernel(T* out, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  double s, c;
+  sincospi(-2.0 * (double)k / (double)n, &s, &c);
+  out[2 * k] = (T)c;
+  out[2 * k + 1] = (T)s;
 }
 
 // w[m] = exp(sign * i * pi * m^2 / n) with m^2 reduced mod 2n in integers.
@@ -114,10 +127,32 @@ struct Plan {
   uint32_t shared;
   uint64_t tw_fwd;
   uint64_t tw_inv;
+  uint64_t roots;
 };
 
 std::mutex g_mu;
 std::map<TwKey, uint64_t> g_tw_cache;  // device-resident twiddle tables, kept for the process
+std::map<std::pair<int, int>, uint64_t> g_roots_cache;
+
+int get_roots(uint64_t* out, int n, int dbl) {
+  *out = 0;
+  if (n <= 0) return 0;
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto key = std::make_pair(n, dbl);
+  auto it = g_roots_cache.find(key);
+  if (it != g_roots_cache.end()) {
+    *out = it->second;
+    return 0;
+  }
+  void* p = nullptr;
+  VKX_HIP(hipMalloc(&p, (size_t)n * (dbl ? 16 : 8)));
+  const uint64_t dptr = reinterpret_cast<uint64_t>(p);
+  if (vkx_roots(dptr, n, dbl, 0) != 0) return 1;
+  VKX_HIP(hipStreamSynchronize(0));
+  g_roots_cache[key] = dptr;
+  *out = dptr;
+  return 0;
+}
 
 int get_twiddles(uint64_t* out, int n, const int* radices, int npass, int dbl) {
   *out = 0;
@@ -365,6 +400,20 @@ int vkx_twiddles(uint64_t dst, int n, const int* radices, int num_passes, int do
   return 0;
 }
 
+int vkx_roots(uint64_t dst, int n, int double_precision, uint64_t stream) {
+  if (n < 1) return fail("vkx_roots", "bad size");
+  const int threads = 256;
+  const int blocks = (n + threads - 1) / threads;
+  if (double_precision)
+    hipLaunchKernelGGL(roots_kernel<double>, dim3(blocks), dim3(threads), 0, S(stream),
+                       reinterpret_cast<double*>(dst), n);
+  else
+    hipLaunchKernelGGL(roots_kernel<float>, dim3(blocks), dim3(threads), 0, S(stream),
+                       reinterpret_cast<float*>(dst), n);
+  VKX_HIP(hipGetLastError());
+  return 0;
+}
+
 int vkx_chirp(uint64_t dst, int n, int count, int sign, int double_precision, uint64_t stream) {
   if (n < 1 || count < 1) return fail("vkx_chirp", "bad size");
   const int threads = 256;
@@ -386,7 +435,10 @@ int vkx_plan_create(uint64_t* plan, const vkx_plan_desc* d) {
   p->fn = reinterpret_cast<hipFunction_t>(d->function);
   for (int i = 0; i < 3; ++i) { p->grid[i] = d->grid[i]; p->block[i] = d->block[i]; }
   p->shared = d->shared_bytes;
-  p->tw_fwd = p->tw_inv = 0;
+  p->tw_fwd = p->tw_inv = p->roots = 0;
+  if (d->roots_n > 0) {
+    if (get_roots(&p->roots, d->roots_n, d->double_precision)) { delete p; return 1; }
+  }
   if (d->n > 1 && d->num_passes_fwd > 0) {
     if (get_twiddles(&p->tw_fwd, d->n, d->radices_fwd, d->num_passes_fwd, d->double_precision)) { delete p; return 1; }
   }
@@ -406,13 +458,14 @@ int vkx_plan_exec(uint64_t plan, const uint64_t* bufs, int num_bufs, const void*
                   uint64_t stream) {
   const Plan* p = reinterpret_cast<const Plan*>(plan);
   if (!p) return fail("vkx_plan_exec", "null plan");
-  if (num_bufs < 0 || num_bufs > kSlotTwFwd) return fail("vkx_plan_exec", "too many buffers (max 14)");
+  if (num_bufs < 0 || num_bufs > kMaxUserBufs) return fail("vkx_plan_exec", "too many buffers (max 12)");
   if (consts_size > (size_t)kParamBytes) return fail("vkx_plan_exec", "constants exceed 256 bytes");
   KParams prm;
   std::memset(&prm, 0, sizeof(prm));
   for (int i = 0; i < num_bufs; ++i) prm.p[i] = bufs[i];
   prm.p[kSlotTwFwd] = p->tw_fwd;
   prm.p[kSlotTwInv] = p->tw_inv;
+  prm.p[kSlotRoots] = p->roots;
   if (consts_size) std::memcpy(prm.c, consts, consts_size);
   size_t sz = sizeof(prm);
   void* cfg[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, &prm, HIP_LAUNCH_PARAM_BUFFER_SIZE, &sz, HIP_LAUNCH_PARAM_END};

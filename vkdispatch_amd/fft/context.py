@@ -83,7 +83,7 @@ class FFTContext:
                 "  __device__ __forceinline__ static void apply(Op& op, const vkx::KParams& prm) {",
                 *self._out_lines, "  }", "};"]
         src = translation_unit(cfg, ["\n".join(fin), "\n".join(fout)], "vkx::fft_body<Cfg, HookIn, HookOut>(prm);",
-                               self.plan.BLOCK)
+                               self.plan.BLOCK, "vkx_fft_context")
         self._kernel = _ContextKernel(src, self.plan, self.args, list(self.plan.radices), None, name=self.name)
         return self._kernel
 

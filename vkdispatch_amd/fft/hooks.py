@@ -122,8 +122,8 @@ def declare_args(output_map, input_map, kernel_map, output_type, input_type, ker
         if not dt.is_dtype(t):
             raise ValueError(f"annotation type {t} is not a vkdispatch_amd dtype")
         if kind == "buffer":
-            if nptr >= 14:
-                raise ValueError("at most 14 buffer arguments per FFT kernel")
+            if nptr >= 12:
+                raise ValueError("at most 12 buffer arguments per FFT kernel")
             args.append(Arg(role, kind, t, nptr))
             nptr += 1
         else:

@@ -13,6 +13,7 @@ from __future__ import annotations
 import ctypes
 import hashlib
 import os
+import re
 import threading
 from typing import Dict, List, Optional, Sequence, Tuple
 
@@ -133,10 +134,26 @@ def cfg_struct(plan: KernelPlan, *, in_os: int, out_os: int, k_os: int, sig: Tup
     return "\n".join(lines)
 
 
-def translation_unit(cfg: str, functors: Sequence[str], body: str, block: int) -> str:
+def kernel_symbol(kind: str, plan: KernelPlan, tag: str = "") -> str:
+    """Descriptive entry-point name (what rocprof reports), e.g. vkx_fft_n4096_s1_o1024_fwd."""
+    sym = f"vkx_{kind}_n{plan.N}_s{plan.S}_o{plan.OUTER}" + (f"_{tag}" if tag else "")
+    return re.sub(r"[^A-Za-z0-9_]", "_", sym)
+
+
+_SYMBOL_RE = re.compile(r'extern "C" __global__ void __launch_bounds__\(\d+\) (\w+)\(')
+
+
+def source_symbol(source: str) -> str:
+    m = _SYMBOL_RE.search(source)
+    if not m:
+        raise ValueError("kernel source has no extern \"C\" entry point")
+    return m.group(1)
+
+
+def translation_unit(cfg: str, functors: Sequence[str], body: str, block: int, symbol: str = "vkx_main") -> str:
     parts = ['#include "vkx_fft.h"', "", cfg, ""]
     parts += list(functors)
-    parts += ["", f'extern "C" __global__ void __launch_bounds__({block}) vkx_main(vkx::KParams prm) {{',
+    parts += ["", f'extern "C" __global__ void __launch_bounds__({block}) {symbol}(vkx::KParams prm) {{',
               f"  {body}", "}", ""]
     return "\n".join(parts)
 
@@ -160,7 +177,8 @@ def load_function(source: str) -> int:
     buf = ctypes.create_string_buffer(code, len(code))
     rt.check(L.vkx_module_load(ctypes.byref(mod), buf, len(code)), "vkx_module_load")
     fn = ctypes.c_uint64(0)
-    rt.check(L.vkx_module_function(ctypes.byref(fn), mod.value, b"vkx_main"), "vkx_module_function")
+    rt.check(L.vkx_module_function(ctypes.byref(fn), mod.value, source_symbol(source).encode()),
+             "vkx_module_function")
     with _mod_lock:
         _modules[key] = (mod.value, fn.value)
     return fn.value
@@ -170,13 +188,14 @@ class CompiledKernel:
     """One FFT / convolution / transpose dispatch: source + plan + argument layout."""
 
     def __init__(self, source: str, plan: KernelPlan, args, twiddle_fwd: Optional[List[int]],
-                 twiddle_inv: Optional[List[int]], name: str = "vkx_fft"):
+                 twiddle_inv: Optional[List[int]], name: str = "vkx_fft", roots_n: int = 0):
         self.source = source
         self.plan = plan
         self.args = list(args)
         self.name = name
         self.tw_fwd = twiddle_fwd
         self.tw_inv = twiddle_inv
+        self.roots_n = int(roots_n)
         self._plan_handle = None
         self._lock = threading.Lock()
 
@@ -208,6 +227,7 @@ class CompiledKernel:
                         d.num_passes_inv = len(self.tw_inv)
                         for i, r in enumerate(self.tw_inv):
                             d.radices_inv[i] = r
+                    d.roots_n = self.roots_n
                     h = ctypes.c_uint64(0)
                     rt.check(rt.lib().vkx_plan_create(ctypes.byref(h), ctypes.byref(d)), "vkx_plan_create")
                     self._plan_handle = h.value

@@ -154,8 +154,58 @@ class KernelPlan:
         return (self.NWG, 1, 1)
 
 
+_OVERRIDES: dict = {}
+_TUNED: Optional[dict] = None
+
+
+def plan_key(geom: AxisGeometry, real: str, kind: str) -> str:
+    """Tuned plans depend on the transform length and stride, not on the batch count."""
+    return f"{kind}|{geom.N}|{geom.S}|{real}"
+
+
+def set_override(geom: AxisGeometry, real: str, kind: str, radices=None, tt=None, B=None) -> None:
+    """Force a plan (used by the autotuner); ``radices=None`` removes the override.
+    Kernel factories are cached: call ``vd.fft.cache_clear()`` after changing overrides."""
+    key = plan_key(geom, real, kind)
+    if radices is None:
+        _OVERRIDES.pop(key, None)
+    else:
+        _OVERRIDES[key] = {"radices": list(radices), "tt": int(tt), "B": None if B is None else int(B)}
+
+
+def _tuned_table() -> dict:
+    global _TUNED
+    if _TUNED is None:
+        import json
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_plans.json")
+        try:
+            with open(path) as f:
+                _TUNED = json.load(f).get("plans", {})
+        except (OSError, ValueError):
+            _TUNED = {}
+    return _TUNED
+
+
+def lookup(geom: AxisGeometry, real: str, kind: str) -> Optional[dict]:
+    key = plan_key(geom, real, kind)
+    if key in _OVERRIDES:
+        return _OVERRIDES[key]
+    if os.environ.get("VKX_NO_TUNED") == "1":
+        return None
+    return _tuned_table().get(key)
+
+
 def make_plan(geom: AxisGeometry, real: str = "float", batch_hint: Optional[int] = None,
-              radices: Optional[List[int]] = None, tt: Optional[int] = None) -> KernelPlan:
+              radices: Optional[List[int]] = None, tt: Optional[int] = None,
+              need_lds: bool = False, kind: str = "fft", B_force: Optional[int] = None) -> KernelPlan:
+    """Plan for ``geom``.  ``need_lds``: the kernel exchanges a full transform through
+    LDS after its passes (real-FFT post-processing), so reserve B padded rows.
+    ``kind`` selects the tuned-plan entry (autotuner results / overrides) if any."""
+    if radices is None:
+        hit = lookup(geom, real, kind)
+        if hit is not None:
+            radices, tt = hit["radices"], hit["tt"]
+            B_force = hit.get("B") or B_force
     N, S, OUTER = geom.N, geom.S, geom.OUTER
     elem = 16 if real == "double" else 8
     strided = S > 1
@@ -188,13 +238,18 @@ def make_plan(geom: AxisGeometry, real: str = "float", batch_hint: Optional[int]
         else:
             B = max(1, 256 // tt)
         B = max(1, min(B, OUTER))
-        while B > 1 and not single_pass and B * NP * elem > LDS_BYTES_TARGET:
+        while B > 1 and (need_lds or not single_pass) and B * NP * elem > LDS_BYTES_TARGET:
             B //= 2
         colfast = False
     if batch_hint is not None:
         B = max(1, min(B, batch_hint))
+    if B_force is not None:
+        B = max(1, min(int(B_force), S if strided else OUTER, max(1, MAX_THREADS // tt)))
+        colfast = strided and B > 1
 
-    if single_pass:
+    if need_lds and not colfast:
+        lds_elems = B * NP
+    elif single_pass:
         lds_elems = 1
     elif colfast:
         lds_elems = N * B

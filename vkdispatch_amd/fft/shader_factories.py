@@ -12,7 +12,7 @@ from typing import Optional, Tuple
 
 from .. import dtypes as dt
 from . import hooks
-from .kernels import CompiledKernel, cfg_struct, translation_unit
+from .kernels import CompiledKernel, cfg_struct, kernel_symbol, translation_unit
 from .planner import AxisGeometry, KernelPlan, make_plan, transposed_size
 from .precision import kernel_real_type
 
@@ -34,7 +34,7 @@ def _resolve_signal_range(signal_range, N: int) -> Tuple[int, int]:
 
 def conv_plan(buffer_shape, axis, compute_type) -> KernelPlan:
     geom = AxisGeometry.of(buffer_shape, axis)
-    return make_plan(geom, kernel_real_type(compute_type))
+    return make_plan(geom, kernel_real_type(compute_type), kind="conv")
 
 
 def _io_functors(args, output_map, input_map, output_type, input_type, compute, N, knum):
@@ -71,6 +71,10 @@ def make_fft_shader(buffer_shape: Tuple, axis: int = None, inverse: bool = False
     N = geom.N
     if r2c and geom.S != 1:
         raise ValueError("R2C io only supported for contiguous data")
+    if (r2c and N % 2 == 0 and input_map is None and output_map is None and output_type is input_type
+            and (not inverse or input_signal_range is None)):
+        return _make_real_row_shader(geom, inverse, normalize_inverse, output_type, compute_type,
+                                     input_signal_range)
     plan = make_plan(geom, kernel_real_type(compute_type))
     args = hooks.declare_args(output_map, input_map, None, output_type, input_type, None, conv=False)
     if r2c:
@@ -83,9 +87,34 @@ def make_fft_shader(buffer_shape: Tuple, axis: int = None, inverse: bool = False
     cfg = cfg_struct(plan, in_os=in_os, out_os=out_os, k_os=N * geom.S, sig=sig, r2c=r2c, inverse=inverse,
                      normalize=normalize, scale=1.0 / N)
     functors = _io_functors(args, output_map, input_map, output_type, input_type, compute, N, 1)
-    src = translation_unit(cfg, functors, "vkx::fft_body<Cfg, HookIn, HookOut>(prm);", plan.BLOCK)
+    kind = ("irfft" if inverse else "rfft") if r2c else "fft"
+    tag = ("inv" if inverse else "fwd") + ("_hook" if (input_map or output_map) else "")
+    src = translation_unit(cfg, functors, "vkx::fft_body<Cfg, HookIn, HookOut>(prm);", plan.BLOCK,
+                           kernel_symbol(kind, plan, tag))
     name = f"fft_shader_{tuple(buffer_shape)}_{axis}_{inverse}_{normalize_inverse}_{r2c}"
     return CompiledKernel(src, plan, args, list(plan.radices), None, name=name)
+
+
+def _make_real_row_shader(geom: AxisGeometry, inverse: bool, normalize_inverse: bool, io_type, compute_type,
+                          input_signal_range) -> CompiledKernel:
+    """r2c / c2r of even-length rows through an N/2-point complex FFT (rfft_body / irfft_body)."""
+    N = geom.N
+    M = N // 2
+    half_geom = AxisGeometry((geom.OUTER, M), 1, M, 1, geom.OUTER)
+    plan = make_plan(half_geom, kernel_real_type(compute_type), need_lds=not inverse,
+                     kind="irfft" if inverse else "rfft")
+    in_os, out_os = (M + 1, 2 * M + 2) if inverse else (2 * M + 2, M + 1)
+    sig = _resolve_signal_range(input_signal_range, N)
+    normalize = bool(inverse and normalize_inverse)
+    cfg = cfg_struct(plan, in_os=in_os, out_os=out_os, k_os=0, sig=sig, r2c=True, inverse=inverse,
+                     normalize=normalize, scale=1.0 / N)
+    from .hooks import Arg
+    args = [Arg("output", "buffer", io_type, 0)]
+    body = f"vkx::{'irfft' if inverse else 'rfft'}_body<Cfg, {io_type.child_type.cpp}>(prm);"
+    kind = "irfft" if inverse else "rfft"
+    src = translation_unit(cfg, [], body, plan.BLOCK, kernel_symbol(kind + "h", plan, "inv" if inverse else "fwd"))
+    return CompiledKernel(src, plan, args, list(plan.radices), None, name=f"{kind}_shader_{geom.shape}",
+                          roots_n=N)
 
 
 @lru_cache(maxsize=None)
@@ -105,7 +134,7 @@ def make_transpose_shader(buffer_shape: Tuple, axis: int = None, kernel_inner_on
                      r2c=False, inverse=False, normalize=False, scale=1.0, k_inner_only=kernel_inner_only,
                      k_transposed=True)
     body = (f"vkx::transpose_body<Cfg, {input_type.child_type.cpp}, {output_type.child_type.cpp}>(prm);")
-    src = translation_unit(cfg, [], body, plan.BLOCK)
+    src = translation_unit(cfg, [], body, plan.BLOCK, kernel_symbol("transpose", plan))
     return CompiledKernel(src, plan, args, None, None, name=f"transpose_shader_{tuple(buffer_shape)}_{axis}")
 
 
@@ -148,7 +177,10 @@ def make_convolution_shader(buffer_shape: Tuple, kernel_map=None, kernel_num: in
                                              per_kernel=True, kernel_num=kernel_num))
     else:
         functors.append(f"using HookKernel = vkx::DefaultKernel<{k_args[0].slot}, {kernel_type.child_type.cpp}>;")
-    src = translation_unit(cfg, functors, "vkx::conv_body<Cfg, HookIn, HookKernel, HookOut>(prm);", plan.BLOCK)
+    tag = ("inner" if kernel_inner_only else "") + ("_t" if transposed_kernel else "") + \
+        ("_hook" if (input_map or output_map or kernel_map) else "") + (f"_k{kernel_num}" if kernel_num > 1 else "")
+    src = translation_unit(cfg, functors, "vkx::conv_body<Cfg, HookIn, HookKernel, HookOut>(prm);", plan.BLOCK,
+                           kernel_symbol("conv", plan, tag))
     return CompiledKernel(src, plan, args, list(plan.radices), list(reversed(plan.radices)),
                           name=f"convolution_shader_{tuple(buffer_shape)}_{axis}")
 

@@ -49,6 +49,7 @@ class PlanDesc(ctypes.Structure):
         ("radices_fwd", ctypes.c_int * VKX_MAX_PASSES),
         ("num_passes_inv", ctypes.c_int),
         ("radices_inv", ctypes.c_int * VKX_MAX_PASSES),
+        ("roots_n", ctypes.c_int),
     ]
 
 
@@ -93,6 +94,7 @@ _SIGS = {
     "vkx_plan_exec": ([_U64, _PU64, ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t, _U64], ctypes.c_int),
     "vkx_twiddles": ([_U64, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int, _U64],
                      ctypes.c_int),
+    "vkx_roots": ([_U64, ctypes.c_int, ctypes.c_int, _U64], ctypes.c_int),
     "vkx_chirp": ([_U64, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _U64], ctypes.c_int),
     "vkx_graph_begin": ([_U64], ctypes.c_int),
     "vkx_graph_end": ([_U64, _PU64], ctypes.c_int),
