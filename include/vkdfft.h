@@ -142,6 +142,10 @@ int vkx_plan_exec(uint64_t plan, const uint64_t* bufs, int num_bufs, const void*
 int vkx_twiddles(uint64_t dst, int n, const int* radices, int num_passes, int double_precision, uint64_t stream);
 /* Roots of unity r[k] = exp(-2 pi i k / n), k in [0, n) (real-FFT pre/post-processing). */
 int vkx_roots(uint64_t dst, int n, int double_precision, uint64_t stream);
+/* dst[i] = sum over b < batch of src[b*count + i] for float (or double) arrays, summed
+ * in a fixed order (deterministic).  No counterpart in the reference: used by the
+ * hand-written convolution adjoint to reduce the kernel gradient over the batch. */
+int vkx_batch_sum(uint64_t dst, uint64_t src, int batch, long long count, int double_precision, uint64_t stream);
 /* Bluestein chirp w[m] = exp(sign * i pi m^2 / n), m in [0, count); m^2 mod 2n in exact integers. */
 int vkx_chirp(uint64_t dst, int n, int count, int sign, int double_precision, uint64_t stream);
 

@@ -671,7 +671,9 @@ __device__ __forceinline__ void conv_body(const KParams& prm) {
 //          (imaginary parts of X[0] and X[M] do not contribute, as in a
 //           complex inverse followed by taking the real part)
 // ---------------------------------------------------------------------------
-template <class C, class S>
+// OOP = true: real rows are read from slot 1 (pitch C::IN_OS reals) and the
+// spectrum written to slot 0; otherwise the RFFTBuffer in slot 0 is used in place.
+template <class C, class S, bool OOP = false>
 __device__ __forceinline__ void rfft_body(const KParams& prm) {
   using real = typename C::real;
   using L = typename C::fwd;
@@ -683,7 +685,7 @@ __device__ __forceinline__ void rfft_body(const KParams& prm) {
   cpx<real> v[C::VMAX];
   const cpx<real>* tw = bufp<const cpx<real>>(prm, SLOT_TW_FWD);
   const cpx<real>* roots = bufp<const cpx<real>>(prm, SLOT_ROOTS);
-  const S* xin = bufp<const S>(prm, 0);
+  const S* xin = bufp<const S>(prm, OOP ? 1 : 0);
   cpx<S>* xout = bufp<cpx<S>>(prm, 0);
 
   if (g.valid) {
@@ -723,7 +725,9 @@ __device__ __forceinline__ void rfft_body(const KParams& prm) {
   }
 }
 
-template <class C, class S>
+// OOP = true: the spectrum is read from slot 1 (pitch C::IN_OS complex) and real
+// rows written to slot 0 (pitch C::OUT_OS reals).
+template <class C, class S, bool OOP = false>
 __device__ __forceinline__ void irfft_body(const KParams& prm) {
   using real = typename C::real;
   using L = typename C::fwd;
@@ -735,7 +739,7 @@ __device__ __forceinline__ void irfft_body(const KParams& prm) {
   cpx<real> v[C::VMAX];
   const cpx<real>* tw = bufp<const cpx<real>>(prm, SLOT_TW_FWD);
   const cpx<real>* roots = bufp<const cpx<real>>(prm, SLOT_ROOTS);
-  const cpx<S>* Xin = bufp<const cpx<S>>(prm, 0);
+  const cpx<S>* Xin = bufp<const cpx<S>>(prm, OOP ? 1 : 0);
   S* xout = bufp<S>(prm, 0);
 
   if (g.valid) {

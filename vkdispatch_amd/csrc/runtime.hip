@@ -93,6 +93,16 @@ __global__ void roots_kernel(T* out, int n) {
   out[2 * k + 1] = (T)s;
 }
 
+// dst[i] = sum_b src[b * count + i] (fixed summation order: deterministic)
+template <typename T>
+__global__ void batch_sum_kernel(T* __restrict__ dst, const T* __restrict__ src, int batch, long long count) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  T acc = src[i];
+  for (int b = 1; b < batch; ++b) acc += src[(long long)b * count + i];
+  dst[i] = acc;
+}
+
 // w[m] = exp(sign * i * pi * m^2 / n) with m^2 reduced mod 2n in integers.
 template <typename T>
 __global__ void chirp_kernel(T* out, int n, int count, int sign) {
@@ -410,6 +420,21 @@ int vkx_roots(uint64_t dst, int n, int double_precision, uint64_t stream) {
   else
     hipLaunchKernelGGL(roots_kernel<float>, dim3(blocks), dim3(threads), 0, S(stream),
                        reinterpret_cast<float*>(dst), n);
+  VKX_HIP(hipGetLastError());
+  return 0;
+}
+
+int vkx_batch_sum(uint64_t dst, uint64_t src, int batch, long long count, int double_precision, uint64_t stream) {
+  if (batch < 1 || count < 0) return fail("vkx_batch_sum", "bad size");
+  if (count == 0) return 0;
+  const int threads = 256;
+  const long long blocks = (count + threads - 1) / threads;
+  if (double_precision)
+    hipLaunchKernelGGL(batch_sum_kernel<double>, dim3((unsigned)blocks), dim3(threads), 0, S(stream),
+                       reinterpret_cast<double*>(dst), reinterpret_cast<const double*>(src), batch, count);
+  else
+    hipLaunchKernelGGL(batch_sum_kernel<float>, dim3((unsigned)blocks), dim3(threads), 0, S(stream),
+                       reinterpret_cast<float*>(dst), reinterpret_cast<const float*>(src), batch, count);
   VKX_HIP(hipGetLastError());
   return 0;
 }

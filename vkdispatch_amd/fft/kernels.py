@@ -287,6 +287,18 @@ def precompile(kernels, workers: int = 0) -> int:
         return 0
     if workers <= 0:
         workers = min(16, os.cpu_count() or 4)
-    with ThreadPoolExecutor(max_workers=workers) as ex:
-        list(ex.map(compile_source, srcs))
+    if workers == 1 or len(srcs) == 1:
+        for s in srcs:
+            compile_source(s)
+        return len(srcs)
+    # hiprtc serialises compilations inside one process: use worker processes
+    # (started with spawn, so they never inherit a GPU context)
+    import multiprocessing as mp
+    from concurrent.futures import ProcessPoolExecutor
+    with ProcessPoolExecutor(max_workers=min(workers, len(srcs)), mp_context=mp.get_context("spawn")) as ex:
+        list(ex.map(_compile_worker, srcs, chunksize=1))
     return len(srcs)
+
+
+def _compile_worker(source: str) -> int:
+    return len(compile_source(source))

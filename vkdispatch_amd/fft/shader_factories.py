@@ -96,25 +96,50 @@ def make_fft_shader(buffer_shape: Tuple, axis: int = None, inverse: bool = False
 
 
 def _make_real_row_shader(geom: AxisGeometry, inverse: bool, normalize_inverse: bool, io_type, compute_type,
-                          input_signal_range) -> CompiledKernel:
-    """r2c / c2r of even-length rows through an N/2-point complex FFT (rfft_body / irfft_body)."""
+                          input_signal_range, oop: bool = False, real_pitch: Optional[int] = None,
+                          spec_pitch: Optional[int] = None) -> CompiledKernel:
+    """r2c / c2r of even-length rows through an N/2-point complex FFT (rfft_body / irfft_body).
+
+    In place on an RFFTBuffer by default.  ``oop``: the real rows live in a separate
+    buffer of row pitch ``real_pitch`` reals (slot 1 for r2c input, slot 0 for c2r
+    output) and the spectrum in one of row pitch ``spec_pitch`` complex values."""
     N = geom.N
     M = N // 2
     half_geom = AxisGeometry((geom.OUTER, M), 1, M, 1, geom.OUTER)
     plan = make_plan(half_geom, kernel_real_type(compute_type), need_lds=not inverse,
                      kind="irfft" if inverse else "rfft")
-    in_os, out_os = (M + 1, 2 * M + 2) if inverse else (2 * M + 2, M + 1)
+    rp = 2 * M + 2 if real_pitch is None else int(real_pitch)
+    sp = M + 1 if spec_pitch is None else int(spec_pitch)
+    in_os, out_os = (sp, rp) if inverse else (rp, sp)
     sig = _resolve_signal_range(input_signal_range, N)
     normalize = bool(inverse and normalize_inverse)
     cfg = cfg_struct(plan, in_os=in_os, out_os=out_os, k_os=0, sig=sig, r2c=True, inverse=inverse,
                      normalize=normalize, scale=1.0 / N)
     from .hooks import Arg
-    args = [Arg("output", "buffer", io_type, 0)]
-    body = f"vkx::{'irfft' if inverse else 'rfft'}_body<Cfg, {io_type.child_type.cpp}>(prm);"
+    if oop:
+        real_t = io_type.child_type
+        args = ([Arg("output", "buffer", real_t, 0), Arg("input", "buffer", io_type, 1)] if inverse else
+                [Arg("output", "buffer", io_type, 0), Arg("input", "buffer", real_t, 1)])
+    else:
+        args = [Arg("output", "buffer", io_type, 0)]
     kind = "irfft" if inverse else "rfft"
-    src = translation_unit(cfg, [], body, plan.BLOCK, kernel_symbol(kind + "h", plan, "inv" if inverse else "fwd"))
+    body = f"vkx::{kind}_body<Cfg, {io_type.child_type.cpp}, {'true' if oop else 'false'}>(prm);"
+    tag = ("inv" if inverse else "fwd") + ("_oop" if oop else "")
+    src = translation_unit(cfg, [], body, plan.BLOCK, kernel_symbol(kind + "h", plan, tag))
     return CompiledKernel(src, plan, args, list(plan.radices), None, name=f"{kind}_shader_{geom.shape}",
                           roots_n=N)
+
+
+@lru_cache(maxsize=None)
+def make_real_rows_shader(real_shape: Tuple, inverse: bool, normalize: bool = True, io_type=dt.complex64,
+                          compute_type=dt.complex64, oop: bool = False, real_pitch: Optional[int] = None,
+                          spec_pitch: Optional[int] = None) -> CompiledKernel:
+    """Row r2c / c2r kernel for even row length (internal building block of fused paths)."""
+    geom = AxisGeometry.of(tuple(real_shape), None)
+    if geom.N % 2:
+        raise ValueError("real-row kernels need an even row length")
+    return _make_real_row_shader(geom, inverse, normalize, io_type, compute_type, None, oop, real_pitch,
+                                 spec_pitch)
 
 
 @lru_cache(maxsize=None)
@@ -205,4 +230,5 @@ def cache_clear():
     make_convolution_shader.cache_clear()
     make_fft_shader.cache_clear()
     make_transpose_shader.cache_clear()
+    make_real_rows_shader.cache_clear()
     get_transposed_size.cache_clear()
