@@ -188,6 +188,45 @@ def bench_c2c(steps, warmup, args):
             "roundtrip_rel_l2": O.rel_l2(res, x)}
 
 
+def bench_ragged(steps, warmup, args):
+    """4096 ragged C2C transforms (60% pow2, 30% 7-smooth, 10% primes), one persistent launch."""
+    import vkdispatch_amd as vd
+    from vkdispatch_amd.fft import ragged as R
+    import oracle as O
+    sizes = R.mixed_sizes(4096, seed=0)
+    parts, packed = R.packed_normal(sizes, 0)
+    src = vd.asbuffer(packed)
+    buf = vd.Buffer((packed.size,), vd.complex64)
+    tuned = R.tune(sizes, buf, reps=3)
+    best = tuned["best"]
+    plan = R.RaggedFFT(sizes, block=best["block"], max_radix=best["max_radix"])
+    g = vd.CommandGraph()
+    g.record_callable(plan.bind(buf, src))
+    g.capture()
+    for _ in range(warmup):
+        g.submit()
+    vd.synchronize()
+    e0, e1 = vd.Event(), vd.Event()
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(steps):
+        g.submit()
+    e1.record()
+    vd.synchronize()
+    wall = (time.perf_counter() - t0) / steps
+    gpu_ms = e0.elapsed_ms(e1) / steps
+    out = buf.read(0)
+    worst, off = 0.0, 0
+    for n, x in zip(sizes, parts):
+        worst = max(worst, O.rel_l2(out[off:off + n], np.fft.fft(x.astype(np.complex64).astype(np.complex128))))
+        off += n
+    alg = 2 * packed.nbytes
+    return {"metric": "ragged 4096 x C2C (64..65536) GFLOP/s", "value": round(plan.flops() / wall / 1e9, 1),
+            "unit": "GFLOP/s", "ms_per_step": round(wall * 1e3, 4), "gpu_ms_per_step": round(gpu_ms, 4),
+            "GB_s": round(alg / wall / 1e9, 1), "points": int(sum(sizes)), "tuned": tuned,
+            "breakdown": plan.breakdown(), "worst_rel_l2_vs_fp64": worst}
+
+
 def _time_kernel(kernel, values, reps):
     import vkdispatch_amd as vd
     e0, e1 = vd.Event(), vd.Event()
@@ -205,7 +244,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="conv2d", choices=["conv2d", "c2c4096"])
+    ap.add_argument("--workload", default="conv2d", choices=["conv2d", "c2c4096", "ragged"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -214,6 +253,8 @@ def main():
         sys.exit(2)
     if args.workload == "conv2d":
         out = bench_conv(args.steps, args.warmup, args)
+    elif args.workload == "ragged":
+        out = bench_ragged(args.steps, args.warmup, args)
     else:
         out = bench_c2c(args.steps, args.warmup, args)
     print(json.dumps(out), flush=True)

@@ -149,6 +149,19 @@ int vkx_batch_sum(uint64_t dst, uint64_t src, int batch, long long count, int do
 /* Bluestein chirp w[m] = exp(sign * i pi m^2 / n), m in [0, count); m^2 mod 2n in exact integers. */
 int vkx_chirp(uint64_t dst, int n, int count, int sign, int double_precision, uint64_t stream);
 
+/* ---- ragged batches ------------------------------------------------------------ */
+/* No counterpart in the reference (its FFTs are one kernel per shape, one length
+ * per launch).  Runs `ntasks` 1-D C2C transform tasks of mixed lengths in one
+ * persistent launch: `tasks` is a device array of 112-byte task records (layout in
+ * vkdispatch_amd/fft/ragged.py, RTASK_DTYPE); `counter` a device int used as the
+ * task queue (reset by the call); `scratch` holds `scratch_per_wg` complex64
+ * elements per workgroup of `grid`; `block` in {64, 128, ..., 512}. */
+int vkx_ragged_exec(uint64_t tasks, int ntasks, uint64_t counter, uint64_t scratch, long long scratch_per_wg,
+                    int grid, int block, uint64_t stream);
+/* Bluestein kernel b[j] = exp(+i pi j^2 / n) for |j| < n (indices mod m), 0 elsewhere,
+ * complex64, phases in FP64 with j^2 reduced mod 2n in integers; m >= 2n-1. */
+int vkx_bluestein_b(uint64_t dst, int n, int m, uint64_t stream);
+
 /* ---- graphs ---------------------------------------------------------------- */
 /* replace vd.CommandGraph recording/submission
  * (vkdispatch/execution_pipeline/command_graph.py:193-316) and its CUDA-graph

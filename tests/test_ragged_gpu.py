@@ -1,0 +1,91 @@
+"""Ragged mixed-size batches, Bluestein and long transforms (GPU), against FP64."""
+import numpy as np
+import pytest
+
+import oracle as O
+import vkdispatch_amd as vd
+from vkdispatch_amd.fft import ragged as R
+
+pytestmark = pytest.mark.gpu
+
+
+def ragged_sizes(count, seed=0, lo=64, hi=65536):
+    return R.mixed_sizes(count, seed=seed, lo=lo, hi=hi)
+
+
+def _packed(sizes, seed):
+    return R.packed_normal(sizes, seed)
+
+
+def _check(sizes, parts, out, inverse=False, tol=1e-5):
+    off = 0
+    worst = 0.0
+    for n, x in zip(sizes, parts):
+        ref = np.fft.ifft(x.astype(np.complex64).astype(np.complex128)) if inverse else \
+            np.fft.fft(x.astype(np.complex64).astype(np.complex128))
+        e = O.rel_l2(out[off:off + n], ref)
+        worst = max(worst, e)
+        assert e <= tol, f"n={n}: rel-L2 {e:.2e}"
+        off += n
+    return worst
+
+
+@pytest.mark.parametrize("n", [67, 97, 101, 257, 1031, 4099, 65521])
+def test_bluestein_primes(n):
+    rng = np.random.default_rng(n)
+    x = (rng.standard_normal((3, n)) + 1j * rng.standard_normal((3, n))).astype(np.complex64)
+    buf = vd.asbuffer(x)
+    vd.fft.fft(buf)
+    assert O.rel_l2(buf.read(0), np.fft.fft(x.astype(np.complex128), axis=1)) < 1e-5
+    vd.fft.ifft(buf)
+    assert O.rel_l2(buf.read(0), x) < 2e-5
+
+
+@pytest.mark.parametrize("n", [32768, 65536, 59049, 2 * 3 * 5 * 7 * 11 * 13 * 2 ** 4, 1 << 20])
+def test_long_transforms(n):
+    rng = np.random.default_rng(n % 1000)
+    x = (rng.standard_normal((2, n)) + 1j * rng.standard_normal((2, n))).astype(np.complex64)
+    buf = vd.asbuffer(x)
+    vd.fft.fft(buf)
+    assert O.rel_l2(buf.read(0), np.fft.fft(x.astype(np.complex128), axis=1)) < 1e-5
+
+
+def test_ragged_batch_small():
+    sizes = ragged_sizes(300, seed=3, hi=8192)
+    parts, packed = _packed(sizes, 4)
+    buf = vd.asbuffer(packed)
+    plan = R.RaggedFFT(sizes)
+    plan(buf)
+    _check(sizes, parts, buf.read(0))
+    inv = R.RaggedFFT(sizes, inverse=True)
+    inv(buf)
+    assert O.rel_l2(buf.read(0), packed) < 1e-5
+
+
+def test_ragged_out_of_place_and_graph():
+    sizes = [64, 100, 4096, 7 * 11 * 13, 97, 65536, 3 ** 8]
+    parts, packed = _packed(sizes, 5)
+    src = vd.asbuffer(packed)
+    dst = vd.Buffer((packed.size,), vd.complex64)
+    plan = R.RaggedFFT(sizes)
+    plan(dst, src)
+    _check(sizes, parts, dst.read(0))
+    assert np.array_equal(src.read(0), packed)
+    g = vd.CommandGraph()
+    g.record_callable(plan.bind(dst, src))
+    for _ in range(3):
+        g.submit()
+        vd.synchronize()
+        _check(sizes, parts, dst.read(0))
+
+
+def test_ragged_4096_mixed_sizes_every_transform():
+    """4096 transforms, lengths 64..65536 (60% pow2, 30% 7-smooth, 10% primes via
+    Bluestein), N(0,1) data from seed 0: every transform within rel-L2 1e-5 of FP64."""
+    sizes = ragged_sizes(4096, seed=0)
+    parts, packed = _packed(sizes, 0)
+    buf = vd.asbuffer(packed)
+    plan = R.RaggedFFT(sizes)
+    plan(buf)
+    worst = _check(sizes, parts, buf.read(0))
+    assert worst < 1e-5

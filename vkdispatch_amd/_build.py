@@ -44,9 +44,10 @@ def hipcc() -> str:
 def build_native(verbose: bool = False) -> str:
     """Compile the runtime library; returns the path of the built .so."""
     os.makedirs(LIB_DIR, exist_ok=True)
-    src = os.path.join(PKG_DIR, "csrc", "runtime.hip")
+    srcs = [os.path.join(PKG_DIR, "csrc", f) for f in ("runtime.hip", "ragged.hip")]
     tmp = LIB_PATH + ".tmp"
-    cmd = [hipcc(), *HIP_FLAGS, "-shared", "-I", os.path.join(REPO_DIR, "include"), src,
+    cmd = [hipcc(), *HIP_FLAGS, "-shared", "-I", os.path.join(REPO_DIR, "include"),
+           "-I", os.path.join(PKG_DIR, "csrc"), *srcs,
            "-o", tmp, f"-L{ROCM}/lib", "-lhiprtc", f"-Wl,-rpath,{ROCM}/lib"]
     if verbose:
         print(" ".join(cmd), flush=True)

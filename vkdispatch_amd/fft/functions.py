@@ -110,9 +110,15 @@ def fft(*buffers: Buffer, buffer_shape: Tuple = None, graph: CommandGraph = None
     else:
         _validate_map_argument_annotations(input_map, "input_map")
     comp_t = prec.resolve_compute_precision(io, compute_type)
-    kernel = make_fft_shader(tuple(buffer_shape), axis, inverse=inverse, normalize_inverse=normalize_inverse,
-                             r2c=r2c, input_map=input_map, output_map=output_map, input_type=in_t,
-                             output_type=out_t, compute_type=comp_t, input_signal_range=input_signal_range)
+    try:
+        kernel = make_fft_shader(tuple(buffer_shape), axis, inverse=inverse, normalize_inverse=normalize_inverse,
+                                 r2c=r2c, input_map=input_map, output_map=output_map, input_type=in_t,
+                                 output_type=out_t, compute_type=comp_t, input_signal_range=input_signal_range)
+    except ValueError:
+        if _large_or_prime_fallback(buffers, buffer_shape, axis, inverse, normalize_inverse, r2c, input_map,
+                                    output_map, out_t, comp_t, input_signal_range, graph):
+            return
+        raise
     if print_shader:
         print(kernel)
     if input_map is None and output_map is None and not r2c:
@@ -122,6 +128,37 @@ def fft(*buffers: Buffer, buffer_shape: Tuple = None, graph: CommandGraph = None
         if buffers[0].size < n:
             raise ValueError(f"buffer of {buffers[0].size} elements is smaller than buffer_shape {tuple(buffer_shape)}")
     dispatch(kernel, buffers, graph)
+
+
+def _large_or_prime_fallback(buffers, buffer_shape, axis, inverse, normalize, r2c, input_map, output_map, out_t,
+                             comp_t, signal_range, graph) -> bool:
+    """Lengths outside the single-workgroup kernels (a prime factor above 31, or more
+    than 16384 points) run on the ragged executor (mixed radix through global scratch,
+    Bluestein for large primes): contiguous axis, complex64, no hooks, in place."""
+    from .planner import AxisGeometry
+    geom = AxisGeometry.of(tuple(buffer_shape), axis)
+    if (geom.S != 1 or r2c or input_map is not None or output_map is not None or signal_range is not None
+            or out_t is not dt.complex64 or comp_t not in (dt.complex64, dt.complex32)):
+        return False
+    from .ragged import RaggedFFT
+    plan = _uniform_ragged(geom.N, geom.OUTER, bool(inverse), bool(normalize))
+    buf = buffers[0]
+    if graph is not None:
+        graph.record_callable(plan.bind(buf))
+    else:
+        plan(buf)
+    return True
+
+
+def _uniform_ragged(n, count, inverse, normalize):
+    from .ragged import RaggedFFT
+    key = (n, count, inverse, normalize)
+    if key not in _UNIFORM_RAGGED:
+        _UNIFORM_RAGGED[key] = RaggedFFT([n] * count, inverse=inverse, normalize=normalize)
+    return _UNIFORM_RAGGED[key]
+
+
+_UNIFORM_RAGGED: dict = {}
 
 
 def fft2(buffer: Buffer, graph: CommandGraph = None, print_shader: bool = False, input_map: MappingFunction = None,
