@@ -227,6 +227,61 @@ def bench_ragged(steps, warmup, args):
             "breakdown": plan.breakdown(), "worst_rel_l2_vs_fp64": worst}
 
 
+def bench_mixed3d(steps, warmup, args):
+    """3-D C2C 256^3: bf16/fp16 storage with MFMA radix-16 stages vs the FP32 VALU path."""
+    import vkdispatch_amd as vd
+    from vkdispatch_amd.fft.mixed import Mixed3D
+    import oracle as O
+    shape = (256, 256, 256)
+    rng = np.random.default_rng(0)
+    z = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+
+    def timed(run):
+        for _ in range(warmup):
+            run()
+        vd.synchronize()
+        e0, e1 = vd.Event(), vd.Event()
+        e0.record()
+        for _ in range(steps):
+            run()
+        e1.record()
+        e1.synchronize()
+        return e0.elapsed_ms(e1) / steps
+
+    res = {}
+    # FP32 baseline: vd.fft.fft3 in place on complex64 (three LDS-butterfly passes)
+    b32 = vd.asbuffer(z)
+    g32 = vd.CommandGraph()
+    vd.fft.fft3(b32, graph=g32)
+    g32.capture()
+    ms32 = timed(g32.submit)
+    b32.write(z)
+    g32.submit()
+    ref_unq = np.fft.fftn(z.astype(np.complex128))
+    res["fp32"] = {"ms": ms32, "rel_l2_vs_fp64": O.rel_l2(b32.read(0), ref_unq),
+                   "bytes": 3 * 2 * z.nbytes}
+    del b32
+    for store, vt in (("bf16", vd.complex_bf16), ("fp16", vd.complex32)):
+        inp = vd.Buffer(shape, vt)
+        inp.write(z)
+        xq = inp.read(0)
+        out = vd.Buffer(shape, vd.complex64)
+        plan = Mixed3D(shape, vt)
+        g = vd.CommandGraph()
+        plan(out, inp, graph=g)
+        g.capture()
+        ms = timed(g.submit)
+        g.submit()
+        r = out.read(0)
+        res[store] = {"ms": ms, "rel_l2_vs_fp64_quantized": O.rel_l2(r, np.fft.fftn(xq.astype(np.complex128))),
+                      "rel_l2_vs_fp64_unquantized": O.rel_l2(r, ref_unq),
+                      "bytes": z.size * (4 + 4 * 4 + 4 + 8), "speedup_vs_fp32": ms32 / ms}
+        del inp, out, plan, g
+    return {"metric": "3-D C2C 256^3: BF16-MFMA split-precision vs FP32 speedup at rel-L2<=1e-3",
+            "value": round(res["bf16"]["speedup_vs_fp32"], 3), "unit": "x", "steps": steps, "warmup": warmup,
+            "detail": res}
+
+
 def _time_kernel(kernel, values, reps):
     import vkdispatch_amd as vd
     e0, e1 = vd.Event(), vd.Event()
@@ -244,7 +299,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="conv2d", choices=["conv2d", "c2c4096", "ragged"])
+    ap.add_argument("--workload", default="conv2d", choices=["conv2d", "c2c4096", "ragged", "mixed3d"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -255,6 +310,8 @@ def main():
         out = bench_conv(args.steps, args.warmup, args)
     elif args.workload == "ragged":
         out = bench_ragged(args.steps, args.warmup, args)
+    elif args.workload == "mixed3d":
+        out = bench_mixed3d(args.steps, args.warmup, args)
     else:
         out = bench_c2c(args.steps, args.warmup, args)
     print(json.dumps(out), flush=True)
