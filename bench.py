@@ -282,6 +282,26 @@ def bench_mixed3d(steps, warmup, args):
             "detail": res}
 
 
+def bench_copy(steps, warmup, args):
+    """Plain device-to-device copy of 128 MiB (calibration of the HBM byte counters)."""
+    import vkdispatch_amd as vd
+    nbytes = 128 << 20
+    a, b = vd.Buffer((nbytes // 8,), vd.complex64), vd.Buffer((nbytes // 8,), vd.complex64)
+    vd.runtime.memset(a.ptr, 1, nbytes)
+    for _ in range(warmup):
+        vd.runtime.dtod(b.ptr, a.ptr, nbytes)
+    vd.synchronize()
+    e0, e1 = vd.Event(), vd.Event()
+    e0.record()
+    for _ in range(steps):
+        vd.runtime.dtod(b.ptr, a.ptr, nbytes)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_ms(e1) / steps
+    return {"metric": "device copy 128 MiB", "value": round(2 * nbytes / (ms * 1e-3) / 1e9, 1), "unit": "GB/s",
+            "ms": ms, "bytes_read": nbytes, "bytes_written": nbytes}
+
+
 def _time_kernel(kernel, values, reps):
     import vkdispatch_amd as vd
     e0, e1 = vd.Event(), vd.Event()
@@ -299,7 +319,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="conv2d", choices=["conv2d", "c2c4096", "ragged", "mixed3d"])
+    ap.add_argument("--workload", default="conv2d", choices=["conv2d", "c2c4096", "ragged", "mixed3d", "copy"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -312,6 +332,8 @@ def main():
         out = bench_ragged(args.steps, args.warmup, args)
     elif args.workload == "mixed3d":
         out = bench_mixed3d(args.steps, args.warmup, args)
+    elif args.workload == "copy":
+        out = bench_copy(args.steps, args.warmup, args)
     else:
         out = bench_c2c(args.steps, args.warmup, args)
     print(json.dumps(out), flush=True)
