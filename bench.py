@@ -87,6 +87,21 @@ def cpu_baseline_conv(x, K, budget_s=12.0):
             "sample": f"{imgs} of {x.shape[0]} images through the FP64 oracle in {dt:.1f} s"}, extra
 
 
+def committed_traffic(kernel_name):
+    """HBM bytes per dispatch of ``kernel_name`` from the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summaries committed under profiles/ (lines 'traffic_bytes <kernel> <bytes>',
+    already scaled by the copy-kernel calibration), or None."""
+    import glob
+    import re
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        with open(path) as f:
+            for line in f:
+                m = re.match(r"traffic_bytes (\S+) (\d+)", line.strip())
+                if m and m.group(1) == kernel_name:
+                    return int(m.group(2))
+    return None
+
+
 def bench_conv(steps, warmup, args):
     import vkdispatch_amd as vd
     from vkdispatch_amd.fft import fused
@@ -136,7 +151,8 @@ def bench_conv(steps, warmup, args):
     dom = max(prof, key=lambda p: p["ms"])
     achieved = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom["name"],
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": committed_traffic(dom["name"]),
+                "kernel": dom["name"],
                 "kernel_ms": round(dom["ms"], 5), "kernel_bytes": dom["bytes"]}
     out = {
         "metric": "FFT-conv GB/s & %HBM roofline; BF16-MFMA vs FP32 speedup at rel-L2<=1e-3",

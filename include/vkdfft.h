@@ -148,6 +148,11 @@ int vkx_roots(uint64_t dst, int n, int double_precision, uint64_t stream);
 int vkx_batch_sum(uint64_t dst, uint64_t src, int batch, long long count, int double_precision, uint64_t stream);
 /* Bluestein chirp w[m] = exp(sign * i pi m^2 / n), m in [0, count); m^2 mod 2n in exact integers. */
 int vkx_chirp(uint64_t dst, int n, int count, int sign, int double_precision, uint64_t stream);
+/* out[w] = XCD (XCC_ID register) that workgroup w of an nwg-workgroup launch ran on
+ * (synchronous, current device).  No counterpart in the reference: the fused 2-D
+ * convolution keeps each image pair on one XCD and checks the workgroup-to-XCD
+ * placement with this probe before it is enabled. */
+int vkx_xcc_map(int* out, int nwg);
 
 /* ---- ragged batches ------------------------------------------------------------ */
 /* No counterpart in the reference (its FFTs are one kernel per shape, one length

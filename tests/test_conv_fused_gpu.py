@@ -1,0 +1,152 @@
+"""Pair-packed 2-D FFT-convolution (convolve2DR with one broadcast kernel spectrum),
+as three phase kernels and as one cooperative launch, against the FP64 oracle and
+against the reference's three-dispatch composition."""
+import contextlib
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+import vkdispatch_amd as vd
+from vkdispatch_amd.fft import fused
+
+from _gpu_common import assert_close
+
+pytestmark = pytest.mark.gpu
+
+
+def _gauss_kernel(rng, n, sigma=4.0):
+    yy, xx = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    dy, dx = np.minimum(yy, n - yy), np.minimum(xx, n - xx)
+    g = np.exp(-(dy ** 2 + dx ** 2) / (2 * sigma ** 2))
+    return (g + 0.01 * rng.standard_normal((n, n))).astype(np.float32)
+
+
+@contextlib.contextmanager
+def _env(name, value):
+    old = os.environ.get(name)
+    os.environ[name] = value
+    try:
+        yield
+    finally:
+        if old is None:
+            del os.environ[name]
+        else:
+            os.environ[name] = old
+
+
+def _run(x, K, kernel_inner_only=True, normalize=True, mode="phases"):
+    buf = vd.asrfftbuffer(x)
+    kb = vd.RFFTBuffer(x.shape[-2:])
+    kb.write_fourier(K)
+    with _env("VKX_CONV2D_MODE", mode):
+        plan = fused.fused_eligible(buf, kb, kernel_inner_only=kernel_inner_only, normalize=normalize)
+        assert plan is not None, "expected the pair-packed path"
+        assert isinstance(plan, fused.Conv2DFused if mode == "fused" else fused.Conv2DPhases)
+        vd.fft.convolve2DR(buf, kb, kernel_inner_only=kernel_inner_only, normalize=normalize)
+    out = buf.read_real(0)
+    if mode == "fused":
+        assert plan.error_bits() == 0
+    return out
+
+
+MODES = ["phases", "fused"]
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("batch,n", [(8, 512), (5, 256), (2, 1024), (1, 512)])
+def test_fused_matches_oracle(batch, n, mode):
+    print(f"pair-packed convolve2DR {mode} batch={batch} n={n}", flush=True)
+    rng = np.random.default_rng(batch * 1000 + n)
+    x = rng.uniform(-1, 1, (batch, n, n)).astype(np.float32)
+    k = _gauss_kernel(rng, n)
+    K = np.fft.rfft2(k.astype(np.float64)).astype(np.complex64)
+    if batch == 1:
+        x = x[0]
+    got = _run(x, K, mode=mode)
+    ref = O.convolve2DR(x, K, kernel_inner_only=True)
+    e = assert_close(got, ref, rel=1e-5)
+    print(f"  rel-L2 {e:.3e}", flush=True)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_fused_non_hermitian_half_spectrum(mode):
+    """A half spectrum that is not Hermitian on the DC and Nyquist columns: the
+    pair-packed path must reproduce what rfft -> convolve -> irfft does with it."""
+    print(f"pair-packed convolve2DR {mode} non-Hermitian K", flush=True)
+    rng = np.random.default_rng(7)
+    n = 256
+    x = rng.uniform(-1, 1, (4, n, n)).astype(np.float32)
+    K = (rng.standard_normal((n, n // 2 + 1)) + 1j * rng.standard_normal((n, n // 2 + 1))).astype(np.complex64)
+    got = _run(x, K, mode=mode)
+    ref = O.convolve2DR(x, K, kernel_inner_only=True)
+    assert_close(got, ref, rel=1e-5)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_fused_unnormalized(mode):
+    print(f"pair-packed convolve2DR {mode} normalize=False", flush=True)
+    rng = np.random.default_rng(8)
+    n = 256
+    x = rng.uniform(-1, 1, (3, n, n)).astype(np.float32)
+    K = np.fft.rfft2(_gauss_kernel(rng, n).astype(np.float64)).astype(np.complex64)
+    got = _run(x, K, normalize=False, mode=mode)
+    ref = O.convolve2DR(x, K, normalize=False, kernel_inner_only=True)
+    assert_close(got, ref, rel=1e-5)
+
+
+def test_fused_agrees_with_three_dispatch_path():
+    print("pair-packed (both modes) vs reference composition 16 x 512^2", flush=True)
+    rng = np.random.default_rng(9)
+    x = rng.uniform(-1, 1, (16, 512, 512)).astype(np.float32)
+    K = np.fft.rfft2(_gauss_kernel(rng, 512).astype(np.float64)).astype(np.complex64)
+    got = _run(x, K, mode="phases")
+    got_f = _run(x, K, mode="fused")
+    assert O.rel_l2(got_f, got) <= 1e-6  # same algorithm; the compiler may contract FMAs differently
+    with _env("VKX_FUSED_CONV", "0"):
+        buf = vd.asrfftbuffer(x)
+        kb = vd.RFFTBuffer((512, 512))
+        kb.write_fourier(K)
+        assert fused.fused_eligible(buf, kb, kernel_inner_only=True) is None
+        vd.fft.convolve2DR(buf, kb, kernel_inner_only=True)
+        three = buf.read_real(0)
+    assert O.rel_l2(got, three) <= 2e-6
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_fused_repeated_launches_and_graph_replays(mode):
+    """Back-to-back launches and graph replays give the same result as the first
+    launch (the cooperative kernel's control block resets itself every launch)."""
+    print(f"pair-packed convolve2DR {mode} repeated launches / graph replays", flush=True)
+    rng = np.random.default_rng(10)
+    n = 512
+    x = rng.uniform(-1, 1, (6, n, n)).astype(np.float32)
+    K = np.fft.rfft2(_gauss_kernel(rng, n).astype(np.float64)).astype(np.complex64)
+    ref = O.convolve2DR(x, K, kernel_inner_only=True)
+    first = _run(x, K, mode=mode)
+    assert_close(first, ref, rel=1e-5)
+    for _ in range(3):
+        assert np.array_equal(_run(x, K, mode=mode), first)
+    # graph: the recorded dispatches replayed on fresh copies of the input
+    buf = vd.asrfftbuffer(x)
+    kb = vd.RFFTBuffer((n, n))
+    kb.write_fourier(K)
+    g = vd.CommandGraph()
+    with _env("VKX_CONV2D_MODE", mode):
+        vd.fft.convolve2DR(buf, kb, kernel_inner_only=True, graph=g)
+        plan = fused.fused_eligible(buf, kb, kernel_inner_only=True)
+    for _ in range(3):
+        buf.write_real(x)
+        g.submit()
+        assert np.array_equal(buf.read_real(0), first)
+    if mode == "fused":
+        assert plan.error_bits() == 0
+
+
+def test_fused_not_used_for_per_image_kernels():
+    """Without kernel_inner_only a batched buffer reads one kernel spectrum per image
+    (the reference's io_index rule): that call keeps the three-dispatch path."""
+    buf = vd.RFFTBuffer((4, 256, 256))
+    kb = vd.RFFTBuffer((4, 256, 256))
+    assert fused.fused_eligible(buf, kb, kernel_inner_only=False) is None

@@ -212,3 +212,15 @@ def test_kernels_compile_for_gfx950(make, tmp_path, monkeypatch):
     monkeypatch.setenv("VKX_KERNEL_CACHE", str(tmp_path))
     code = K.compile_source(make().source)
     assert code[:4] == b"\x7fELF"
+
+
+def test_pair_conv_kernels_do_not_spill():
+    """Every standard pair-packed convolution kernel keeps its values in registers
+    (a spilling 16-wave variant gave wrong rows on the MI355X; the plans refuse
+    spilling kernels at launch, so the defaults must not spill)."""
+    from vkdispatch_amd.fft import fused
+    for batch, n, norm in fused._STANDARD_FUSED:
+        for mode in ("phases", "fused"):
+            for k in fused.fused_plan(batch, n, n, norm, mode).kernels():
+                info = k.code_object_info()
+                assert info["private_bytes"] == 0, (k.name, info)

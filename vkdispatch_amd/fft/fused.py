@@ -1,20 +1,319 @@
 """2-D real FFT-convolution (convolve2DR) execution paths and their profiling.
 
-``try_convolve2DR`` returns False when the call is not eligible for a fused
-path, and the caller falls back to the reference's three-dispatch composition
-(rfft -> convolve(axis ndim-2) -> irfft, vkdispatch/fft/functions.py:614-646).
+``try_convolve2DR`` runs the single-pass kernel (``csrc/device/vkx_conv2d.h``)
+when the call is eligible: a batch of square real images (256, 512 or 1024 on a
+side) in complex64 RFFTBuffers, one kernel spectrum broadcast over the batch,
+the default kernel map.  It returns False otherwise, and the caller falls back
+to the reference's three-dispatch composition (rfft -> convolve(axis ndim-2) ->
+irfft, vkdispatch/fft/functions.py:614-646).  Both paths compute
+irfft2(rfft2(x) . conj(K)) per image (vkdispatch/fft/shader_factories.py:136-145).
+
+The single-pass kernel keeps every image pair on one XCD (one task queue per
+XCD, chosen by the XCC_ID register of the workgroup).  Set VKX_FUSED_CONV=0 to
+force the three-dispatch path.
 """
 from __future__ import annotations
 
-from typing import Dict, List
+import ctypes
+import os
+import threading
+from functools import lru_cache
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
 
 from .. import dtypes as dt
 from .. import runtime as rt
+from ..buffer import Buffer
+from ..graph import dispatch
+from .hooks import Arg
+from .kernels import CompiledKernel, kernel_symbol, source_symbol
+from .planner import KernelPlan
 from .shader_factories import make_convolution_shader, make_fft_shader
 
+NUM_XCD = 8
+_FUSED_RADICES = {256: (4, 4, 4, 4), 512: (8, 8, 8), 1024: (16, 16, 4)}
+_PHASE_A, _PHASE_B, _PHASE_C, _PHASE_K = 0, 1, 2, 3
+_CTL_ERROR = 512
 
-def try_convolve2DR(buffer, kernel, **kw) -> bool:
-    return False
+
+def _fused_enabled() -> bool:
+    return os.environ.get("VKX_FUSED_CONV", "1") != "0"
+
+
+_placement: Dict[Tuple[int, int], bool] = {}
+_placement_lock = threading.Lock()
+
+
+def xcd_placement_ok(nwg: int) -> bool:
+    """True when workgroup w of an ``nwg``-workgroup launch of a small probe kernel runs
+    on XCD w % 8 (XCC_ID register; diagnostic only: the single-pass kernel picks its
+    queue from the register and does not depend on the placement)."""
+    rt.ensure_device()
+    key = (0, int(nwg))
+    with _placement_lock:
+        if key not in _placement:
+            out = (ctypes.c_int * nwg)()
+            rt.check(rt.lib().vkx_xcc_map(out, nwg), "vkx_xcc_map")
+            _placement[key] = all(out[w] == w % NUM_XCD for w in range(nwg))
+        return _placement[key]
+
+
+def task_table(npairs: int, na: int, nb: int, nk: int, lag_b: int = 1, lag_c: int = 2) -> np.ndarray:
+    """Per-XCD task queues: words 0..8 are queue offsets, then task words
+    (phase << 30 | pair << 12 | slice).  Each queue starts with the ``nk``
+    kernel-prep tasks of its own Kt copy; pair p runs on queue p % 8; a queue's pairs
+    are software-pipelined: step t issues A(t), B(t - lag_b), C(t - lag_c)."""
+    if not (0 < lag_b < lag_c):
+        raise ValueError("need 0 < lag_b < lag_c")
+    queues: List[List[int]] = []
+    for q in range(NUM_XCD):
+        pairs = list(range(q, npairs, NUM_XCD))
+        words: List[int] = [(_PHASE_K << 30) | s for s in range(nk)] if pairs else []
+        for t in range(len(pairs) + lag_c):
+            for phase, lag, n in ((_PHASE_A, 0, na), (_PHASE_B, lag_b, nb), (_PHASE_C, lag_c, na)):
+                i = t - lag
+                if 0 <= i < len(pairs):
+                    words += [(phase << 30) | (pairs[i] << 12) | s for s in range(n)]
+        queues.append(words)
+    offs = np.cumsum([0] + [len(w) for w in queues]).astype(np.uint32)
+    return np.concatenate([offs, np.array([w for ws in queues for w in ws], dtype=np.uint32)])
+
+
+COL_TILE = 16      # columns per column task (128-byte tile rows)
+KPREP_BLOCK = 64   # kernel-prep task: 64 x 64 elements of Kt
+
+
+def _conv_cfg(batch: int, H: int, W: int, normalize: bool, waves: int, rows: int) -> Tuple[str, KernelPlan]:
+    """``struct Cfg`` of the cooperative / phase kernels and the matching KernelPlan
+    (radices, twiddle tables)."""
+    if H != W or W not in _FUSED_RADICES:
+        raise ValueError("the pair-packed 2-D convolution takes square 256, 512 or 1024 images")
+    if waves not in (1, 2, 4, 8, 16) or rows % waves or H % rows:
+        raise ValueError("tiling does not divide the image")
+    radices = _FUSED_RADICES[W]
+    NP = W + W // 32 + 1  # odd stride between LDS column regions: conflict-free tile transposes
+    pitch = 2 * (W // 2 + 1)
+    npairs = (batch + 1) // 2
+    scale = 1.0 / (H * W) if normalize else 1.0
+    plan = KernelPlan(real="float", N=W, TT=64, B=waves, S=1, OUTER=batch, radices=tuple(radices), COLFAST=False,
+                      WAVE_LOCAL=True, PAD_SHIFT=5, NP=NP, LDS_ELEMS=max(COL_TILE * NP, waves * NP), NWG=1,
+                      NWG_INNER=1)
+    pl = lambda rs: "vkx::PL<" + ", ".join(f"vkx::P<{r}, {ns}>" for r, ns in rs) + ">"  # noqa: E731
+    cfg = "\n".join([
+        "struct Cfg {",
+        f"  static constexpr int H = {H}, W = {W}, NW = {waves}, RA = {rows}, NP = {NP};",
+        f"  static constexpr vkx::u32 PITCH = {pitch}u, KW = {W // 2 + 1}u, NIMG = {batch}u, NPAIRS = {npairs}u;",
+        f"  static constexpr vkx::u64 IMG = {H * pitch}ull;",
+        f"  static constexpr double SCALE = {scale!r};",
+        f"  using fwd = {pl(plan.fwd)};",
+        f"  using inv = {pl(plan.inv)};",
+        "};",
+    ])
+    return cfg, plan
+
+
+def _check_no_spill(kernels: Sequence[CompiledKernel]) -> None:
+    """The pair-packed kernels are used only when they keep everything in registers:
+    a 16-wave variant that spilled to private memory produced wrong rows in its
+    fourth wave slot per SIMD on the MI355X, so any spilling variant is refused."""
+    for k in kernels:
+        spill = k.code_object_info()["private_bytes"]
+        if spill:
+            raise RuntimeError(f"{k.name}: {spill} bytes/lane of register spills; choose another tiling")
+
+
+def _conv_tu(cfg: str, symbol: str, block: int, waves_per_simd: Optional[int], body: str) -> str:
+    occ = f" __attribute__((amdgpu_waves_per_eu({waves_per_simd}, {waves_per_simd})))" if waves_per_simd else ""
+    return "\n".join(['#include "vkx_conv2d.h"', "", cfg, "",
+                      f'extern "C" __global__ void __launch_bounds__({block}){occ} {symbol}(vkx::KParams prm) {{',
+                      f"  {body}", "}", ""])
+
+
+class Conv2DFused:
+    """Single-pass (cooperative) plan for ``batch`` real H x W images (H == W) convolved
+    with one broadcast half spectrum.  ``waves`` waves per workgroup, ``rows`` rows per
+    row task; column tasks take 16 columns; ``wg_per_xcd`` workgroups per XCD."""
+
+    def __init__(self, batch: int, H: int, W: int, normalize: bool = True, waves: int = 8, rows: int = 16,
+                 wg_per_xcd: int = 32, lag_b: int = 1, lag_c: int = 2, waves_per_simd: Optional[int] = None):
+        cfg, plan = _conv_cfg(batch, H, W, normalize, waves, rows)
+        self.batch, self.H, self.W = int(batch), int(H), int(W)
+        self.npairs = (self.batch + 1) // 2
+        self.na = H // rows
+        self.nb = W // COL_TILE
+        self.nk = (H // KPREP_BLOCK) * (W // KPREP_BLOCK)
+        grid = NUM_XCD * wg_per_xcd
+        self.plan = KernelPlan(**{**plan.__dict__, "NWG": grid})
+        symbol = kernel_symbol("conv2d", self.plan, "fused")
+        src = _conv_tu(cfg, symbol, 64 * waves, waves_per_simd, "vkx::conv2d_pair_body<Cfg>(prm);")
+        args = [Arg("output", "buffer", dt.complex64, 0), Arg("kernel", "buffer", dt.complex64, 1),
+                Arg("scratch", "buffer", dt.complex64, 2), Arg("control", "buffer", dt.uint32, 3),
+                Arg("tasks", "buffer", dt.uint32, 4), Arg("kernel_t", "buffer", dt.complex64, 5)]
+        radices = list(self.plan.radices)
+        self.kernel = CompiledKernel(src, self.plan, args, radices, radices[::-1],
+                                     name=f"conv2d_fused_{batch}x{H}x{W}")
+        self.table = task_table(self.npairs, self.na, self.nb, self.nk, lag_b, lag_c)
+        self.ctl_words = 1024 + 64 * self.npairs
+        self._dev: Optional[Tuple[Buffer, Buffer, Buffer, Buffer]] = None
+        self._checked = False
+        self._lock = threading.Lock()
+
+    @property
+    def grid(self) -> int:
+        return self.plan.NWG
+
+    def kernels(self) -> List[CompiledKernel]:
+        return [self.kernel]
+
+    def resources(self) -> Tuple[Buffer, Buffer, Buffer, Buffer]:
+        """(scratch, control, task table, per-queue Kt); the control block starts at zero
+        and every launch leaves it at zero (error bits aside)."""
+        if self._dev is None:
+            with self._lock:
+                if self._dev is None:
+                    scratch = Buffer((self.npairs * self.H, self.W), dt.complex64)
+                    ctl = Buffer((self.ctl_words,), dt.uint32)
+                    ctl.write(np.zeros(self.ctl_words, np.uint32))
+                    table = Buffer((self.table.size,), dt.uint32)
+                    table.write(self.table)
+                    kt = Buffer((NUM_XCD * self.H, self.W), dt.complex64)
+                    self._dev = (scratch, ctl, table, kt)
+        return self._dev
+
+    def values(self, buffer, kernel) -> list:
+        if not self._checked:
+            _check_no_spill(self.kernels())
+            self._checked = True
+        scratch, ctl, table, kt = self.resources()
+        return [buffer, kernel, scratch, ctl, table, kt]
+
+    def __call__(self, buffer, kernel, graph=None) -> None:
+        dispatch(self.kernel, self.values(buffer, kernel), graph)
+
+    def error_bits(self) -> int:
+        """Error word of the control block (bit 0: a dependency wait timed out; bit 2: a
+        queue had no workgroup on its XCD and the last workgroup ran it, which is
+        correct but slow).  Synchronises the device."""
+        ctl = self.resources()[1]
+        rt.synchronize()
+        return int(ctl.read(0)[_CTL_ERROR])
+
+    def clear_error(self) -> None:
+        ctl = self.resources()[1]
+        rt.synchronize()
+        ctl.write(np.zeros(self.ctl_words, np.uint32))
+
+
+class Conv2DPhases:
+    """The same computation as three ordinary kernels in stream order: row FFTs of the
+    image pairs (plus the kernel prep), column FFT * Kt * IFFT, row IFFTs; each pair's
+    spectrum goes through a column-major scratch image (64 MiB for 64 images of
+    512 x 512) and Kt is one 512 x 512 complex image."""
+
+    def __init__(self, batch: int, H: int, W: int, normalize: bool = True, waves: int = 8, rows: int = 16,
+                 col_waves: int = 8,
+                 waves_per_simd: Tuple[Optional[int], Optional[int], Optional[int]] = (None, None, None)):
+        self.batch, self.H, self.W = int(batch), int(H), int(W)
+        self.npairs = (self.batch + 1) // 2
+        self.kernels_ = []
+        na, nb = H // rows, W // COL_TILE
+        nk = (H // KPREP_BLOCK) * (W // KPREP_BLOCK)
+        for phase, tag in enumerate(("rows", "cols", "irows")):
+            nw = col_waves if phase == 1 else waves
+            cfg, plan = _conv_cfg(batch, H, W, normalize, nw, rows if phase != 1 else 16)
+            nwg = (self.npairs * na + nk, self.npairs * nb, self.npairs * na)[phase]
+            plan = KernelPlan(**{**plan.__dict__, "NWG": nwg})
+            symbol = kernel_symbol("conv2d", plan, f"pairs_{tag}")
+            src = _conv_tu(cfg, symbol, 64 * nw, waves_per_simd[phase],
+                           f"vkx::conv2d_phase_body<Cfg, {phase}>(prm);")
+            args = [Arg("output", "buffer", dt.complex64, 0), Arg("kernel", "buffer", dt.complex64, 1),
+                    Arg("scratch", "buffer", dt.complex64, 2), Arg("unused", "buffer", dt.complex64, 3),
+                    Arg("unused", "buffer", dt.complex64, 4), Arg("kernel_t", "buffer", dt.complex64, 5)]
+            radices = list(plan.radices)
+            self.kernels_.append(CompiledKernel(src, plan, args, radices, radices[::-1],
+                                                name=f"conv2d_{tag}_{batch}x{H}x{W}"))
+        self._dev: Optional[Tuple[Buffer, Buffer]] = None
+        self._checked = False
+        self._lock = threading.Lock()
+
+    def kernels(self) -> List[CompiledKernel]:
+        return list(self.kernels_)
+
+    def resources(self) -> Tuple[Buffer, Buffer]:
+        if self._dev is None:
+            with self._lock:
+                if self._dev is None:
+                    self._dev = (Buffer((self.npairs * self.H, self.W), dt.complex64),
+                                 Buffer((self.H, self.W), dt.complex64))
+        return self._dev
+
+    def scratch(self) -> Buffer:
+        return self.resources()[0]
+
+    def values(self, buffer, kernel) -> list:
+        if not self._checked:
+            _check_no_spill(self.kernels())
+            self._checked = True
+        scr, kt = self.resources()
+        return [buffer, kernel, scr, scr, scr, kt]
+
+    def __call__(self, buffer, kernel, graph=None) -> None:
+        vals = self.values(buffer, kernel)
+        for k in self.kernels_:
+            dispatch(k, vals, graph)
+
+
+def _mode() -> str:
+    """VKX_CONV2D_MODE: "phases" (three kernels, default), "fused" (one cooperative
+    launch), or VKX_FUSED_CONV=0 for the reference's three-dispatch composition."""
+    return os.environ.get("VKX_CONV2D_MODE", "phases")
+
+
+# cooperative-kernel tilings that keep every value in registers (no private memory)
+_FUSED_TILING = {1024: dict(waves=4, rows=8)}
+
+
+@lru_cache(maxsize=32)
+def fused_plan(batch: int, H: int, W: int, normalize: bool = True, mode: str = "phases"):
+    if mode == "fused":
+        return Conv2DFused(batch, H, W, normalize, **_FUSED_TILING.get(W, {}))
+    return Conv2DPhases(batch, H, W, normalize)
+
+
+def fused_eligible(buffer, kernel, kernel_map=None, buffer_shape=None, transposed_kernel=False,
+                   kernel_inner_only=False, normalize=True, compute_type=None, **_) -> Optional[Conv2DFused]:
+    """The single-pass plan for this convolve2DR call, or None when the call needs the
+    three-dispatch composition."""
+    if not _fused_enabled() or kernel_map is not None or transposed_kernel:
+        return None
+    if compute_type is not None and compute_type is not dt.complex64:
+        return None
+    if getattr(buffer, "var_type", None) is not dt.complex64 or getattr(kernel, "var_type", None) is not dt.complex64:
+        return None
+    rs = tuple(getattr(buffer, "real_shape", ()))
+    if len(rs) not in (2, 3) or (buffer_shape is not None and tuple(buffer_shape) != tuple(buffer.shape)):
+        return None
+    H, W = rs[-2], rs[-1]
+    batch = rs[0] if len(rs) == 3 else 1
+    if H != W or W not in _FUSED_RADICES:
+        return None
+    wh = W // 2 + 1
+    if batch > 1 and not kernel_inner_only:
+        return None  # the reference then reads one kernel spectrum per image
+    if kernel.size < H * wh:
+        return None
+    if batch > 1 and tuple(kernel.shape[-2:]) != (H, wh):
+        return None
+    return fused_plan(batch, H, W, bool(normalize), _mode())
+
+
+def try_convolve2DR(buffer, kernel, graph=None, **kw) -> bool:
+    plan = fused_eligible(buffer, kernel, **kw)
+    if plan is None:
+        return False
+    plan(buffer, kernel, graph)
+    return True
 
 
 def composed_kernels(real_shape, kernel_inner_only=True, compute_type=dt.complex64):
@@ -30,8 +329,16 @@ def composed_kernels(real_shape, kernel_inner_only=True, compute_type=dt.complex
     return r, c, i
 
 
+_STANDARD_FUSED = ((64, 512, True), (8, 512, True), (16, 512, True), (6, 512, True), (4, 512, True),
+                   (1, 512, True), (5, 256, True), (4, 256, True), (3, 256, False), (2, 1024, True))
+
+
 def standard_kernels() -> List:
-    return list(composed_kernels((64, 512, 512)))
+    ks = list(composed_kernels((64, 512, 512))) + list(composed_kernels((16, 512, 512)))
+    for batch, n, norm in _STANDARD_FUSED:
+        for mode in ("phases", "fused"):
+            ks += fused_plan(batch, n, n, norm, mode).kernels()
+    return ks
 
 
 def profile_convolve2DR(buffer, kernel, reps: int = 20) -> List[Dict]:
@@ -43,15 +350,29 @@ def profile_convolve2DR(buffer, kernel, reps: int = 20) -> List[Dict]:
         batch *= s
     H, W = shape[-2], shape[-1]
     Wh = W // 2 + 1
-    r, c, i = composed_kernels(shape)
     real_b = batch * H * W * 4
     spec_b = batch * H * Wh * 8
     kb = H * Wh * 8
+    plan = fused_eligible(buffer, kernel, kernel_inner_only=True)
+    if plan is not None:
+        vals = plan.values(buffer, kernel)
+        ks = plan.kernels()
+        if len(ks) == 1:
+            return [{"name": source_symbol(ks[0].source), "ms": _event_ms(ks[0], vals, reps),
+                     "bytes": 2 * real_b + kb}]
+        cplx_b = (batch + 1) // 2 * H * W * 8
+        kt_b = H * W * 8
+        return [{"name": source_symbol(ks[0].source), "ms": _event_ms(ks[0], vals, reps),
+                 "bytes": real_b + cplx_b + kb + kt_b},
+                {"name": source_symbol(ks[1].source), "ms": _event_ms(ks[1], vals, reps),
+                 "bytes": 2 * cplx_b + kt_b},
+                {"name": source_symbol(ks[2].source), "ms": _event_ms(ks[2], vals, reps),
+                 "bytes": cplx_b + real_b}]
+    r, c, i = composed_kernels(shape)
     out = []
-    for name, k, vals, nbytes in (("rfft_rows", r, [buffer], real_b + spec_b),
-                                  ("convolve_cols", c, [buffer, kernel], 2 * spec_b + kb),
-                                  ("irfft_rows", i, [buffer], spec_b + real_b)):
-        out.append({"name": name, "ms": _event_ms(k, vals, reps), "bytes": nbytes})
+    for k, vals, nbytes in ((r, [buffer], real_b + spec_b), (c, [buffer, kernel], 2 * spec_b + kb),
+                            (i, [buffer], spec_b + real_b)):
+        out.append({"name": source_symbol(k.source), "ms": _event_ms(k, vals, reps), "bytes": nbytes})
     return out
 
 

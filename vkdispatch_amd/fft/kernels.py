@@ -140,7 +140,7 @@ def kernel_symbol(kind: str, plan: KernelPlan, tag: str = "") -> str:
     return re.sub(r"[^A-Za-z0-9_]", "_", sym)
 
 
-_SYMBOL_RE = re.compile(r'extern "C" __global__ void __launch_bounds__\(\d+\) (\w+)\(')
+_SYMBOL_RE = re.compile(r'extern "C" __global__ void __launch_bounds__\(\d+\)(?: __attribute__\(\(\w+\([\d, ]+\)\)\))? (\w+)\(')
 
 
 def source_symbol(source: str) -> str:
@@ -233,6 +233,19 @@ class CompiledKernel:
                     self._plan_handle = h.value
         return self._plan_handle
 
+    def code_object_info(self) -> Dict[str, int]:
+        """Resource figures from the compiled code object's metadata note (no device
+        needed): VGPRs, SGPRs, static LDS bytes and private (spill) bytes per lane."""
+        return code_object_info(compile_source(self.source))
+
+    def attributes(self) -> Dict[str, int]:
+        """Register count, static LDS, maximum block size and private (spill) bytes per
+        lane of the loaded code object."""
+        fn = load_function(self.source)
+        vals = [ctypes.c_int(0) for _ in range(4)]
+        rt.check(rt.lib().vkx_function_attrs(fn, *[ctypes.byref(v) for v in vals]), "vkx_function_attrs")
+        return dict(zip(("num_regs", "shared_bytes", "max_threads", "private_bytes"), (v.value for v in vals)))
+
     def bind(self, values: Sequence) -> Tuple[List[int], bytes]:
         """Device pointers and packed constants for positional ``values``."""
         if len(values) != len(self.args):
@@ -271,6 +284,33 @@ class CompiledKernel:
         cbuf = ctypes.create_string_buffer(consts, max(1, len(consts)))
         s = rt.current_stream() if stream is None else int(stream)
         rt.check(rt.lib().vkx_plan_exec(h, arr, len(ptrs), cbuf, len(consts), s), f"{self.name} launch")
+
+
+def code_object_info(code: bytes) -> Dict[str, int]:
+    """Kernel resource figures from the NT_AMDGPU_METADATA note (msgpack) of an
+    AMDGPU code object built here."""
+    import struct
+
+    import msgpack
+    shoff = struct.unpack_from("<Q", code, 0x28)[0]
+    shentsize, shnum = struct.unpack_from("<HH", code, 0x3A)
+    for i in range(shnum):
+        off = shoff + i * shentsize
+        if struct.unpack_from("<I", code, off + 4)[0] != 7:  # SHT_NOTE
+            continue
+        sh_offset, sh_size = struct.unpack_from("<QQ", code, off + 0x18)
+        pos = sh_offset
+        while pos + 12 <= sh_offset + sh_size:
+            namesz, descsz, ntype = struct.unpack_from("<III", code, pos)
+            dpos = pos + 12 + ((namesz + 3) & ~3)
+            if ntype == 32:
+                md = msgpack.unpackb(code[dpos:dpos + descsz], raw=False)
+                k = md["amdhsa.kernels"][0]
+                return {"vgprs": int(k.get(".vgpr_count", 0)), "sgprs": int(k.get(".sgpr_count", 0)),
+                        "lds_bytes": int(k.get(".group_segment_fixed_size", 0)),
+                        "private_bytes": int(k.get(".private_segment_fixed_size", 0))}
+            pos = dpos + ((descsz + 3) & ~3)
+    raise ValueError("code object has no AMDGPU metadata note")
 
 
 def precompile(kernels, workers: int = 0) -> int:
