@@ -111,7 +111,10 @@ __device__ __forceinline__ void signal_done(u32* c) {
 // C: H, W (= N of the wave transforms), PITCH (floats per real row), IMG (floats
 // per image), NIMG, NPAIRS, NW (waves per workgroup), RA (rows per row task), NP
 // (padded LDS region of one transform; odd so tile transposes are conflict-free),
-// KW = W/2+1, SCALE, fwd / inv (Stockham pass lists, inv = reversed fwd).
+// KW = W/2+1, SCALE, fwd / inv (Stockham pass lists, inv = reversed fwd), LAYOUT
+// (0: tile-major scratch, 1: column-major), CB (columns per wave, column-major),
+// PP (pairs per column task, column-major), TP (LDS stride per column of a
+// row-block transpose tile, > RA).
 template <class C> struct ConvArgs {
   float* x;
   const cpx<float>* K;
@@ -156,7 +159,10 @@ __device__ __forceinline__ void conv_kprep(const cpx<float>* K, cpx<float>* kt, 
   using SH = ConvShape<C>;
   const u32 ky0 = (slice / (C::W / SH::KB)) * SH::KB, kx0 = (slice % (C::W / SH::KB)) * SH::KB;
   const float sc = (float)C::SCALE;
-  for (int e = tid; e < (int)(SH::KB * SH::KB); e += SH::NT) {
+  static_assert(SH::KB * SH::KB % SH::NT == 0, "kernel-prep split");
+#pragma unroll
+  for (int i = 0; i < (int)(SH::KB * SH::KB) / SH::NT; ++i) {
+    const int e = tid + i * SH::NT;
     const u32 ky = ky0 + (u32)e % SH::KB, kx = kx0 + (u32)e / SH::KB;
     const u32 kym = (C::H - ky) & (C::H - 1);
     const bool lo = kx <= (u32)(C::W / 2);
@@ -215,9 +221,17 @@ __device__ __forceinline__ void conv_cols(const ConvArgs<C>& a, const cpx<float>
   using FL = typename pl_last<L>::type;
   using IL = typename pl_last<LI>::type;
   cpx<float>* T = a.scr + (u64)pair * (u64)(C::H * C::W) + slice * (u32)(16 * C::H);
-  for (int e = tid; e < 16 * C::H; e += SH::NT) {
-    const int y = e >> 4, c = e & 15;
-    lds[c * C::NP + y + (y >> 5)] = ld_c64<L2ONLY>(T + e);
+  constexpr int IT = 16 * C::H / SH::NT;  // tile elements per thread
+  static_assert(16 * C::H % SH::NT == 0, "tile split");
+  {
+    cpx<float> t[IT];  // all loads in flight before the first LDS write
+#pragma unroll
+    for (int i = 0; i < IT; ++i) t[i] = ld_c64<L2ONLY>(T + tid + i * SH::NT);
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+      const int e = tid + i * SH::NT, y = e >> 4, c = e & 15;
+      lds[c * C::NP + y + (y >> 5)] = t[i];
+    }
   }
   __syncthreads();
 #pragma nounroll
@@ -238,8 +252,9 @@ __device__ __forceinline__ void conv_cols(const ConvArgs<C>& a, const cpx<float>
     for_layout<WC, IL::R>(lane, [&](auto slot, u32 n) { col[n + (n >> 5)] = v[decltype(slot)::value]; });
   }
   __syncthreads();
-  for (int e = tid; e < 16 * C::H; e += SH::NT) {
-    const int y = e >> 4, c = e & 15;
+#pragma unroll
+  for (int i = 0; i < IT; ++i) {
+    const int e = tid + i * SH::NT, y = e >> 4, c = e & 15;
     T[e] = lds[c * C::NP + y + (y >> 5)];
   }
 }
@@ -280,6 +295,134 @@ __device__ __forceinline__ void conv_rows_inv(const ConvArgs<C>& a, cpx<float>* 
   }
 }
 
+// ---- column-major layout (S[kx * H + ky]): the transposes move into the row
+// kernels (LDS tile of W x TP, RA-element runs per column) and the column kernel
+// streams whole contiguous columns, each wave on its own CB columns. ----
+template <class C>
+__device__ __forceinline__ void conv_rows_fwd_cm(const ConvArgs<C>& a, cpx<float>* lds, u32 pair, u32 slice,
+                                                 int tid, int wave, int lane) {
+  using SH = ConvShape<C>;
+  using WC = WaveCfg<C::W, C::NP>;
+  using L = typename C::fwd;
+  using F0 = typename pl_first<L>::type;
+  using FL = typename pl_last<L>::type;
+  constexpr int IT = C::W * C::RA / SH::NT;
+  static_assert(C::W * C::RA % SH::NT == 0 && C::TP > C::RA, "transpose tile");
+  const u32 ia = 2 * pair, ib = 2 * pair + 1;
+  const bool has_b = ib < C::NIMG;
+  const u32 y0 = slice * C::RA;
+  cpx<float>* S = a.scr + (u64)pair * (u64)(C::H * C::W);
+  cpx<float> v[SH::RPW][SH::NV];
+#pragma unroll
+  for (int i = 0; i < SH::RPW; ++i) {
+    const u32 y = y0 + (u32)(wave + C::NW * i);
+    const float* ra = a.x + (u64)ia * C::IMG + y * C::PITCH;
+    const float* rb = a.x + (u64)ib * C::IMG + y * C::PITCH;
+    for_layout<WC, F0::R>(lane, [&](auto slot, u32 n) {
+      v[i][decltype(slot)::value] = cpx<float>{ra[n], has_b ? rb[n] : 0.f};
+    });
+  }
+#pragma unroll
+  for (int i = 0; i < SH::RPW; ++i) run_passes<WC, -1>(L{}, v[i], lds + wave * C::NP, lane, 0, a.twf);
+  __syncthreads();  // the transpose tile overlaps the exchange regions
+#pragma unroll
+  for (int i = 0; i < SH::RPW; ++i) {
+    const u32 yl = (u32)(wave + C::NW * i);
+    for_layout<WC, FL::R>(lane, [&](auto slot, u32 kx) { lds[kx * C::TP + yl] = v[i][decltype(slot)::value]; });
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < IT; ++i) {
+    const u32 e = (u32)(tid + i * SH::NT), yl = e % C::RA, kx = e / C::RA;
+    S[kx * C::H + y0 + yl] = lds[kx * C::TP + yl];
+  }
+}
+
+// Column task: CB columns per wave for PP consecutive pairs; each wave loads its
+// Kt columns once and reuses them for every pair of the task.
+template <class C, bool L2ONLY>
+__device__ __forceinline__ void conv_cols_cm(const ConvArgs<C>& a, const cpx<float>* kt, cpx<float>* lds, u32 pair0,
+                                             u32 slice, int wave, int lane) {
+  using SH = ConvShape<C>;
+  using WC = WaveCfg<C::W, C::NP>;
+  using L = typename C::fwd;
+  using LI = typename C::inv;
+  using F0 = typename pl_first<L>::type;
+  using FL = typename pl_last<L>::type;
+  using IL = typename pl_last<LI>::type;
+  const u32 kx0 = slice * (u32)(C::NW * C::CB) + (u32)(wave * C::CB);
+  cpx<float>* xch = lds + wave * C::NP;
+#pragma unroll
+  for (int cc = 0; cc < C::CB; ++cc) {
+    const u32 kx = kx0 + (u32)cc;
+    cpx<float> kv[SH::NV];
+    for_layout<WC, FL::R>(lane, [&](auto slot, u32 ky) { kv[decltype(slot)::value] = ld_c64<L2ONLY>(kt + kx * C::H + ky); });
+#pragma nounroll
+    for (u32 pp = 0; pp < (u32)C::PP; ++pp) {
+      const u32 pair = pair0 + pp;
+      if (pair >= C::NPAIRS) break;
+      cpx<float>* col = a.scr + (u64)pair * (u64)(C::H * C::W) + kx * C::H;
+      cpx<float> v[SH::NV];
+      for_layout<WC, F0::R>(lane, [&](auto slot, u32 n) { v[decltype(slot)::value] = ld_c64<L2ONLY>(col + n); });
+      run_passes<WC, -1>(L{}, v, xch, lane, 0, a.twf);
+#pragma unroll
+      for (int i = 0; i < SH::NV; ++i) v[i] = cmul(v[i], kv[i]);
+      block_sync<WC>();
+      run_passes<WC, 1>(LI{}, v, xch, lane, 0, a.twi);
+      for_layout<WC, IL::R>(lane, [&](auto slot, u32 n) { col[n] = v[decltype(slot)::value]; });
+      block_sync<WC>();
+    }
+  }
+}
+
+template <class C, bool L2ONLY>
+__device__ __forceinline__ void conv_rows_inv_cm(const ConvArgs<C>& a, cpx<float>* lds, u32 pair, u32 slice,
+                                                 int tid, int wave, int lane) {
+  using SH = ConvShape<C>;
+  using WC = WaveCfg<C::W, C::NP>;
+  using LI = typename C::inv;
+  using I0 = typename pl_first<LI>::type;
+  using IL = typename pl_last<LI>::type;
+  constexpr int IT = C::W * C::RA / SH::NT;
+  const u32 ia = 2 * pair, ib = 2 * pair + 1;
+  const bool has_b = ib < C::NIMG;
+  const u32 y0 = slice * C::RA;
+  const cpx<float>* S = a.scr + (u64)pair * (u64)(C::H * C::W);
+  {
+    cpx<float> t[IT];  // all loads in flight before the first LDS write
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+      const u32 e = (u32)(tid + i * SH::NT), yl = e % C::RA, kx = e / C::RA;
+      t[i] = ld_c64<L2ONLY>(S + kx * C::H + y0 + yl);
+    }
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+      const u32 e = (u32)(tid + i * SH::NT), yl = e % C::RA, kx = e / C::RA;
+      lds[kx * C::TP + yl] = t[i];
+    }
+  }
+  __syncthreads();
+  cpx<float> v[SH::RPW][SH::NV];
+#pragma unroll
+  for (int i = 0; i < SH::RPW; ++i) {
+    const u32 yl = (u32)(wave + C::NW * i);
+    for_layout<WC, I0::R>(lane, [&](auto slot, u32 kx) { v[i][decltype(slot)::value] = lds[kx * C::TP + yl]; });
+  }
+  __syncthreads();  // the exchange regions overlap the transpose tile
+#pragma unroll
+  for (int i = 0; i < SH::RPW; ++i) {
+    const u32 y = y0 + (u32)(wave + C::NW * i);
+    run_passes<WC, 1>(LI{}, v[i], lds + wave * C::NP, lane, 0, a.twi);
+    float* ra = a.x + (u64)ia * C::IMG + y * C::PITCH;
+    float* rb = a.x + (u64)ib * C::IMG + y * C::PITCH;
+    for_layout<WC, IL::R>(lane, [&](auto slot, u32 n) {
+      const cpx<float> z = v[i][decltype(slot)::value];
+      ra[n] = z.x;
+      if (has_b) rb[n] = z.y;
+    });
+  }
+}
+
 // One phase over all pairs as an ordinary (dependency-free) kernel.  PHASE 0: the
 // row-FFT tasks of every pair (workgroup w < NPAIRS*NA), then the NK kernel-prep
 // tasks; PHASE 1: column tiles, tile-major over the workgroups (workgroups that run
@@ -287,7 +430,11 @@ __device__ __forceinline__ void conv_rows_inv(const ConvArgs<C>& a, cpx<float>* 
 template <class C, int PHASE>
 __device__ __forceinline__ void conv2d_phase_body(const KParams& prm) {
   using SH = ConvShape<C>;
-  constexpr int LDS_ELEMS = PHASE == 1 ? SH::TILE : SH::XCH;
+  constexpr bool CM = C::LAYOUT == 1;
+  constexpr int ROWS_LDS = CM ? (C::W * C::TP > SH::XCH ? C::W * C::TP : SH::XCH) : SH::XCH;
+  constexpr int LDS_ELEMS = PHASE == 1 ? (CM ? SH::XCH : SH::TILE) : ROWS_LDS;
+  constexpr u32 NBT = CM ? (u32)(C::W / (C::NW * C::CB)) : SH::NB;  // column tasks per pair (group)
+  constexpr u32 NPG = CM ? (C::NPAIRS + C::PP - 1) / C::PP : C::NPAIRS;  // pair groups
   __shared__ cpx<float> lds[LDS_ELEMS];
   const ConvArgs<C> a = conv_args<C>(prm);
   const int tid = (int)threadIdx.x;
@@ -295,12 +442,22 @@ __device__ __forceinline__ void conv2d_phase_body(const KParams& prm) {
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const u32 wg = blockIdx.x;
   if constexpr (PHASE == 0) {
-    if (wg < C::NPAIRS * SH::NA) conv_rows_fwd<C>(a, lds, wg / SH::NA, wg % SH::NA, wave, lane);
-    else if (wg < C::NPAIRS * SH::NA + SH::NK) conv_kprep<C>(a.K, a.kt, wg - C::NPAIRS * SH::NA, tid);
+    if (wg < C::NPAIRS * SH::NA) {
+      if constexpr (CM) conv_rows_fwd_cm<C>(a, lds, wg / SH::NA, wg % SH::NA, tid, wave, lane);
+      else conv_rows_fwd<C>(a, lds, wg / SH::NA, wg % SH::NA, wave, lane);
+    } else if (wg < C::NPAIRS * SH::NA + SH::NK) {
+      conv_kprep<C>(a.K, a.kt, wg - C::NPAIRS * SH::NA, tid);
+    }
   } else if constexpr (PHASE == 1) {
-    if (wg < C::NPAIRS * SH::NB) conv_cols<C, false>(a, a.kt, lds, wg % C::NPAIRS, wg / C::NPAIRS, tid, wave, lane);
+    if (wg < NPG * NBT) {
+      if constexpr (CM) conv_cols_cm<C, false>(a, a.kt, lds, (wg % NPG) * C::PP, wg / NPG, wave, lane);
+      else conv_cols<C, false>(a, a.kt, lds, wg % C::NPAIRS, wg / C::NPAIRS, tid, wave, lane);
+    }
   } else {
-    if (wg < C::NPAIRS * SH::NA) conv_rows_inv<C, false>(a, lds, wg / SH::NA, wg % SH::NA, wave, lane);
+    if (wg < C::NPAIRS * SH::NA) {
+      if constexpr (CM) conv_rows_inv_cm<C, false>(a, lds, wg / SH::NA, wg % SH::NA, tid, wave, lane);
+      else conv_rows_inv<C, false>(a, lds, wg / SH::NA, wg % SH::NA, wave, lane);
+    }
   }
 }
 
@@ -309,6 +466,7 @@ __device__ __forceinline__ void conv2d_phase_body(const KParams& prm) {
 template <class C>
 __device__ __forceinline__ void conv2d_pair_body(const KParams& prm) {
   using SH = ConvShape<C>;
+  static_assert(C::LAYOUT == 0, "the cooperative kernel uses the tile-major layout");
   __shared__ cpx<float> lds[SH::LDS_ALL];
   __shared__ u32 s_task;
   const ConvArgs<C> a = conv_args<C>(prm);

@@ -279,7 +279,13 @@ constexpr int SLOT_ROOTS = 13;   // exp(-2 pi i k / roots_n), bound by the plan
 constexpr int SLOT_TW_FWD = 14;
 constexpr int SLOT_TW_INV = 15;
 
-template <class T> __device__ __forceinline__ T* bufp(const KParams& prm, int slot) { return (T*)prm.p[slot]; }
+// Buffer pointers are device (global) memory: the cast through the global address
+// space lets the compiler emit global_* instead of flat_* memory instructions (flat
+// accesses also count against the LDS wait counter, which couples LDS waits to
+// outstanding global loads).
+template <class T> __device__ __forceinline__ T* bufp(const KParams& prm, int slot) {
+  return (T*)(__attribute__((address_space(1))) T*)prm.p[slot];
+}
 template <class T> __device__ __forceinline__ T cval(const KParams& prm, int off) { return *(const T*)(prm.c + off); }
 
 // ---------------------------------------------------------------------------

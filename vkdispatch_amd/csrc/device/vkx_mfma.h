@@ -63,18 +63,29 @@ __device__ __forceinline__ void fft256_mfma_body(const KParams& prm) {
   }
   const u32 count = C::S > 1 ? C::S : C::OUTER;
 
-  // ---- load tile into LDS (FP32) ----
-  for (int idx = tid; idx < T * N; idx += C::BLOCK) {
-    int c, n;
-    if constexpr (C::S > 1) { c = idx % T; n = idx / T; }
-    else { c = idx / N; n = idx % N; }
-    const u32 t = first + (u32)c;
-    cpx<float> v{0.f, 0.f};
-    if (t < count) {
+  // ---- load tile into LDS (FP32); every load is issued before the first LDS write ----
+  constexpr int IT = T * N / C::BLOCK;
+  static_assert(T * N % C::BLOCK == 0, "tile split");
+  {
+    cpx<SIn> raw[IT];
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+      const int idx = tid + i * C::BLOCK;
+      int c, n;
+      if constexpr (C::S > 1) { c = idx % T; n = idx / T; }
+      else { c = idx / N; n = idx % N; }
+      const u32 t = first + (u32)c;
       const u32 gi = C::S > 1 ? outer * C::IN_OS + t + (u32)n * C::S : t * C::IN_OS + (u32)n;
-      v = ccvt<float>(in[gi]);
+      raw[i] = t < count ? in[gi] : cpx<SIn>{};
     }
-    lds[c * LP + n] = v;
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+      const int idx = tid + i * C::BLOCK;
+      int c, n;
+      if constexpr (C::S > 1) { c = idx % T; n = idx / T; }
+      else { c = idx / N; n = idx % N; }
+      lds[c * LP + n] = ccvt<float>(raw[i]);
+    }
   }
 
   // ---- per-lane constants: DFT-16 operands (hi/lo) and stage twiddles ----
@@ -155,7 +166,9 @@ __device__ __forceinline__ void fft256_mfma_body(const KParams& prm) {
 
   // ---- store tile ----
   const float scale = (float)C::OUT_SCALE;
-  for (int idx = tid; idx < T * N; idx += C::BLOCK) {
+#pragma unroll
+  for (int i = 0; i < IT; ++i) {
+    const int idx = tid + i * C::BLOCK;
     int c, n;
     if constexpr (C::S > 1) { c = idx % T; n = idx / T; }
     else { c = idx / N; n = idx % N; }

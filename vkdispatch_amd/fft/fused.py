@@ -84,7 +84,8 @@ COL_TILE = 16      # columns per column task (128-byte tile rows)
 KPREP_BLOCK = 64   # kernel-prep task: 64 x 64 elements of Kt
 
 
-def _conv_cfg(batch: int, H: int, W: int, normalize: bool, waves: int, rows: int) -> Tuple[str, KernelPlan]:
+def _conv_cfg(batch: int, H: int, W: int, normalize: bool, waves: int, rows: int, layout: int = 0,
+              cols_per_wave: int = 1, pairs_per_task: int = 1) -> Tuple[str, KernelPlan]:
     """``struct Cfg`` of the cooperative / phase kernels and the matching KernelPlan
     (radices, twiddle tables)."""
     if H != W or W not in _FUSED_RADICES:
@@ -102,7 +103,8 @@ def _conv_cfg(batch: int, H: int, W: int, normalize: bool, waves: int, rows: int
     pl = lambda rs: "vkx::PL<" + ", ".join(f"vkx::P<{r}, {ns}>" for r, ns in rs) + ">"  # noqa: E731
     cfg = "\n".join([
         "struct Cfg {",
-        f"  static constexpr int H = {H}, W = {W}, NW = {waves}, RA = {rows}, NP = {NP};",
+        f"  static constexpr int H = {H}, W = {W}, NW = {waves}, RA = {rows}, NP = {NP}, LAYOUT = {layout}, "
+        f"CB = {cols_per_wave}, PP = {pairs_per_task}, TP = {rows + 1};",
         f"  static constexpr vkx::u32 PITCH = {pitch}u, KW = {W // 2 + 1}u, NIMG = {batch}u, NPAIRS = {npairs}u;",
         f"  static constexpr vkx::u64 IMG = {H * pitch}ull;",
         f"  static constexpr double SCALE = {scale!r};",
@@ -212,19 +214,24 @@ class Conv2DPhases:
     512 x 512) and Kt is one 512 x 512 complex image."""
 
     def __init__(self, batch: int, H: int, W: int, normalize: bool = True, waves: int = 8, rows: int = 16,
-                 col_waves: int = 8,
+                 col_waves: int = 8, layout: str = "column", cols_per_wave: int = 1, pairs_per_task: int = 4,
                  waves_per_simd: Tuple[Optional[int], Optional[int], Optional[int]] = (None, None, None)):
         self.batch, self.H, self.W = int(batch), int(H), int(W)
         self.npairs = (self.batch + 1) // 2
+        self.layout = layout
+        lay = {"tile": 0, "column": 1}[layout]
         self.kernels_ = []
-        na, nb = H // rows, W // COL_TILE
+        na = H // rows
+        pp = pairs_per_task if lay == 1 else 1
+        nb = W // COL_TILE if lay == 0 else W // (col_waves * cols_per_wave)
+        npg = (self.npairs + pp - 1) // pp
         nk = (H // KPREP_BLOCK) * (W // KPREP_BLOCK)
         for phase, tag in enumerate(("rows", "cols", "irows")):
             nw = col_waves if phase == 1 else waves
-            cfg, plan = _conv_cfg(batch, H, W, normalize, nw, rows if phase != 1 else 16)
-            nwg = (self.npairs * na + nk, self.npairs * nb, self.npairs * na)[phase]
+            cfg, plan = _conv_cfg(batch, H, W, normalize, nw, rows if phase != 1 else 16, lay, cols_per_wave, pp)
+            nwg = (self.npairs * na + nk, npg * nb, self.npairs * na)[phase]
             plan = KernelPlan(**{**plan.__dict__, "NWG": nwg})
-            symbol = kernel_symbol("conv2d", plan, f"pairs_{tag}")
+            symbol = kernel_symbol("conv2d", plan, f"pairs_{tag}" + ("_cm" if lay else ""))
             src = _conv_tu(cfg, symbol, 64 * nw, waves_per_simd[phase],
                            f"vkx::conv2d_phase_body<Cfg, {phase}>(prm);")
             args = [Arg("output", "buffer", dt.complex64, 0), Arg("kernel", "buffer", dt.complex64, 1),
