@@ -291,7 +291,8 @@ def bench_mixed3d(steps, warmup, args):
         r = out.read(0)
         res[store] = {"ms": ms, "rel_l2_vs_fp64_quantized": O.rel_l2(r, np.fft.fftn(xq.astype(np.complex128))),
                       "rel_l2_vs_fp64_unquantized": O.rel_l2(r, ref_unq),
-                      "bytes": z.size * (4 + 4 * 4 + 4 + 8), "speedup_vs_fp32": ms32 / ms}
+                      "passes": plan.npasses, "bytes": z.size * (4 + 8 * (plan.npasses - 1) + 8),
+                      "speedup_vs_fp32": ms32 / ms}
         del inp, out, plan, g
     return {"metric": "3-D C2C 256^3: BF16-MFMA split-precision vs FP32 speedup at rel-L2<=1e-3",
             "value": round(res["bf16"]["speedup_vs_fp32"], 3), "unit": "x", "steps": steps, "warmup": warmup,

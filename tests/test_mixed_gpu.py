@@ -7,7 +7,7 @@ import pytest
 import oracle as O
 import vkdispatch_amd as vd
 from vkdispatch_amd import dtypes
-from vkdispatch_amd.fft.mixed import fft3_mixed
+from vkdispatch_amd.fft.mixed import Mixed3D, fft3_mixed
 
 pytestmark = pytest.mark.gpu
 
@@ -34,20 +34,53 @@ def test_bf16_rounding_matches_torch():
     assert np.array_equal(ours, ref)
 
 
+@pytest.fixture(scope="module")
+def references(volume):
+    """FP64 transforms of the bf16- and fp16-quantized volume (the values actually stored)."""
+    out = {}
+    for store, vt in (("bf16", vd.complex_bf16), ("fp16", vd.complex32)):
+        inp = vd.Buffer(SHAPE, vt)
+        inp.write(volume)
+        xq = inp.read(0).astype(np.complex128)
+        out[store] = (inp, np.fft.fftn(xq), np.fft.ifftn(xq))
+    return out
+
+
+@pytest.mark.parametrize("passes", [2, 3])
 @pytest.mark.parametrize("store", ["bf16", "fp16"])
-def test_fft3_mixed_256cubed(volume, store):
-    vt = vd.complex_bf16 if store == "bf16" else vd.complex32
-    inp = vd.Buffer(SHAPE, vt)
-    inp.write(volume)
-    xq = inp.read(0)                       # the quantized input actually stored
+def test_fft3_mixed_256cubed(volume, references, store, passes):
+    inp, ref, _ = references[store]
     out = vd.Buffer(SHAPE, vd.complex64)
-    fft3_mixed(out, inp)
+    Mixed3D(SHAPE, inp.var_type, passes=passes)(out, inp)
     res = out.read(0)
-    ref = np.fft.fftn(xq.astype(np.complex128))
     err = O.rel_l2(res, ref)
-    err_unq = O.rel_l2(res, np.fft.fftn(volume.astype(np.complex128)))
-    print(f"{store}: rel-L2 vs FP64(quantized) {err:.3e}, vs FP64(unquantized) {err_unq:.3e}")
-    assert err <= (1e-3 if store == "bf16" else 5e-4)  # fp16 intermediates: 2 roundings at 11 bits
+    err_unq = O.rel_l2(res, np.fft.fftn(volume.astype(np.complex128))) if passes == 2 else float("nan")
+    print(f"{store} {passes}-pass: rel-L2 vs FP64(quantized) {err:.3e}, vs FP64(unquantized) {err_unq:.3e}")
+    # fp16 intermediates: one rounding at 11 bits (two-pass) or two (three-pass)
+    assert err <= (1e-3 if store == "bf16" else 5e-4)
+
+
+@pytest.mark.parametrize("passes", [2, 3])
+def test_fft3_mixed_inverse_normalized(references, passes):
+    inp, _, refi = references["bf16"]
+    out = vd.Buffer(SHAPE, vd.complex64)
+    Mixed3D(SHAPE, inp.var_type, inverse=True, passes=passes)(out, inp)
+    assert O.rel_l2(out.read(0), refi) <= 1e-3
+
+
+def test_fft3_mixed_two_pass_repeat_and_graph(references):
+    """The intermediate is fully rewritten each call: repeated calls and graph replays agree."""
+    inp, ref, _ = references["bf16"]
+    plan = Mixed3D(SHAPE, inp.var_type)
+    out = vd.Buffer(SHAPE, vd.complex64)
+    plan(out, inp)
+    first = out.read(0)
+    assert O.rel_l2(first, ref) <= 1e-3
+    g = vd.CommandGraph()
+    plan(out, inp, graph=g)
+    for _ in range(3):
+        g.submit()
+        assert np.array_equal(out.read(0), first)
 
 
 def test_fft3_mixed_inverse_small_values(volume):

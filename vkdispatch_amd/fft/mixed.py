@@ -60,40 +60,88 @@ def make_tw_pass(nb: int, in_type, out_type, inverse: bool = False, out_scale: f
     return CompiledKernel(src, plan, args, None, None, name=f"mfma256t_{nb}", roots_n=256)
 
 
+def _mfma_tu(cfg: str, body: str, block: int, symbol: str, waves_per_simd: Optional[int]) -> str:
+    occ = f" __attribute__((amdgpu_waves_per_eu({waves_per_simd}, {waves_per_simd})))" if waves_per_simd else ""
+    return "\n".join(['#include "vkx_mfma.h"', "", cfg, "",
+                      f'extern "C" __global__ void __launch_bounds__({block}){occ} {symbol}(vkx::KParams prm) {{',
+                      f"  {body}", "}", ""])
+
+
+def make_cube_pass(which: int, in_type, inverse: bool = False, out_scale: float = 1.0, prefetch: int = 4,
+                   waves_per_simd: Optional[int] = 2, tiles: Optional[int] = None) -> CompiledKernel:
+    """One pass of the two-pass 256^3 transform (``fft3_p1_body`` / ``fft3_p2_body``):
+    pass 1 reads the 16-bit volume (slot 1) and writes the fp16 intermediate (slot 0),
+    pass 2 reads the intermediate (slot 1) and writes complex64 in natural order (slot 0).
+    1024 workgroups of 256 threads each."""
+    if which == 1:
+        if in_type not in (dt.complex_bf16, dt.complex32):
+            raise ValueError("pass 1 takes complex_bf16 / complex32 input")
+        out_type = dt.complex32
+        body = f"vkx::fft3_p1_body<Cfg, {_SCALAR_CPP[in_type]}>(prm);"
+    elif which == 2:
+        in_type, out_type = dt.complex32, dt.complex64
+        body = "vkx::fft3_p2_body<Cfg>(prm);"
+    else:
+        raise ValueError("pass is 1 or 2")
+    block, nwg = 256, 1024
+    if tiles is None:
+        tiles = 2 if which == 1 else 1  # tiles per plane exchange (LDS: 2 workgroups per CU)
+    plan = KernelPlan(real="float", N=256, TT=block, B=1, S=1, OUTER=256 * 256, radices=(16, 16, 16), COLFAST=False,
+                      WAVE_LOCAL=True, PAD_SHIFT=5, NP=0, LDS_ELEMS=0, NWG=nwg, NWG_INNER=1)
+    cfg = "\n".join([
+        "struct Cfg {",
+        f"  static constexpr int BLOCK = {block}, DIR = {1 if inverse else -1}, PF = {int(prefetch)}, NT = {int(tiles)};",
+        f"  static constexpr double OUT_SCALE = {out_scale!r};",
+        "};",
+    ])
+    tag = f"p{which}_" + ("inv" if inverse else "fwd") + f"_{in_type.name}"
+    src = _mfma_tu(cfg, body, block, kernel_symbol("mfma3d", plan, tag), waves_per_simd)
+    args = [Arg("output", "buffer", out_type, 0), Arg("input", "buffer", in_type, 1)]
+    return CompiledKernel(src, plan, args, None, None, name=f"mfma3d_p{which}", roots_n=256)
+
+
 class Mixed3D:
-    """Plan for the 3-D mixed-precision transform of one 256 x 256 x 256 volume: three
-    transposed-write MFMA passes (each along the then-contiguous axis), 16-bit storage
-    in between, complex64 result in natural order."""
+    """Plan for the 3-D mixed-precision transform of one 256 x 256 x 256 volume, complex64
+    result in natural order.  ``passes=2`` (default): the two-pass form, 16-bit input ->
+    fp16 intermediate -> complex64 (three radix-16 MFMA stages per pass).  ``passes=3``:
+    three transposed-write 256-point passes with 16-bit storage in between."""
 
     def __init__(self, shape, in_type=dt.complex_bf16, inverse: bool = False, normalize: bool = True,
-                 block: int = 256, T16: Optional[int] = None):
+                 block: int = 256, T16: Optional[int] = None, passes: int = 2):
         shape = tuple(int(s) for s in shape)
         if len(shape) != 3 or any(s != 256 for s in shape):
             raise ValueError("fft3_mixed handles 256 x 256 x 256 volumes")
         if in_type not in (dt.complex_bf16, dt.complex32):
             raise ValueError("input storage must be complex_bf16 or complex32")
+        if passes not in (2, 3):
+            raise ValueError("passes is 2 or 3")
         self.shape = shape
         self.in_type = in_type
         self.inverse = inverse
-        nb = 256 * 256
-        s_mid = 1.0 / 16.0
-        s_last = 256.0
-        if inverse and normalize:
-            s_last = 256.0 / float(256 ** 3)
+        self.npasses = passes
+        norm = 1.0 / float(256 ** 3) if (inverse and normalize) else 1.0
         mid = dt.complex32
-        self.passes = [
-            make_tw_pass(nb, in_type, mid, inverse, s_mid, T16, block),
-            make_tw_pass(nb, mid, mid, inverse, s_mid, T16, block),
-            make_tw_pass(nb, mid, dt.complex64, inverse, s_last, None, block),
-        ]
+        if passes == 2:
+            # 1/64 after pass 1 (three radix-16 stages) keeps N(0,1)-scale data near unit size in fp16
+            self.passes = [make_cube_pass(1, in_type, inverse, 1.0 / 64.0),
+                           make_cube_pass(2, mid, inverse, 64.0 * norm)]
+        else:
+            nb = 256 * 256
+            s_mid = 1.0 / 16.0
+            self.passes = [
+                make_tw_pass(nb, in_type, mid, inverse, s_mid, T16, block),
+                make_tw_pass(nb, mid, mid, inverse, s_mid, T16, block),
+                make_tw_pass(nb, mid, dt.complex64, inverse, 256.0 * norm, None, block),
+            ]
         self._scratch: Optional[Tuple[Buffer, Buffer]] = None
 
     def kernels(self):
         return list(self.passes)
 
-    def scratch(self) -> Tuple[Buffer, Buffer]:
+    def scratch(self) -> Tuple[Buffer, ...]:
         if self._scratch is None:
-            self._scratch = (Buffer(self.shape, dt.complex32), Buffer(self.shape, dt.complex32))
+            n = 1 if self.npasses == 2 else 2
+            self._scratch = tuple(Buffer(self.shape, dt.complex32) for _ in range(n))
         return self._scratch
 
     def __call__(self, out: Buffer, inp: Buffer, graph: Optional[CommandGraph] = None) -> None:
@@ -101,6 +149,11 @@ class Mixed3D:
             raise ValueError("output must be a complex64 buffer of the volume's shape")
         if inp.var_type is not self.in_type or tuple(inp.shape) != self.shape:
             raise ValueError(f"input must be a {self.in_type.name} buffer of the volume's shape")
+        if self.npasses == 2:
+            (m,) = self.scratch()
+            dispatch(self.passes[0], [m, inp], graph)    # -> M[c1][k2/4][i0][a1][k2%4]
+            dispatch(self.passes[1], [out, m], graph)    # -> [k0][k1][k2]
+            return
         s1, s2 = self.scratch()
         dispatch(self.passes[0], [s1, inp], graph)   # [i0][i1][i2] -> [k2][i0][i1]
         dispatch(self.passes[1], [s2, s1], graph)    # -> [k1][k2][i0]
