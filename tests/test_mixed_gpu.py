@@ -56,7 +56,8 @@ def test_fft3_mixed_256cubed(volume, references, store, passes):
     err = O.rel_l2(res, ref)
     err_unq = O.rel_l2(res, np.fft.fftn(volume.astype(np.complex128))) if passes == 2 else float("nan")
     print(f"{store} {passes}-pass: rel-L2 vs FP64(quantized) {err:.3e}, vs FP64(unquantized) {err_unq:.3e}")
-    # fp16 intermediates: one rounding at 11 bits (two-pass) or two (three-pass)
+    # two 11-bit roundings: the fp16 intermediate plus pass 1's fp16 tile results
+    # (two-pass), or the two fp16 intermediates (three-pass)
     assert err <= (1e-3 if store == "bf16" else 5e-4)
 
 
@@ -66,6 +67,14 @@ def test_fft3_mixed_inverse_normalized(references, passes):
     out = vd.Buffer(SHAPE, vd.complex64)
     Mixed3D(SHAPE, inp.var_type, inverse=True, passes=passes)(out, inp)
     assert O.rel_l2(out.read(0), refi) <= 1e-3
+
+
+def test_fft3_mixed_two_pass_runs_of_eight(references):
+    """Pass 2 with eight consecutive k2 per workgroup (one 8-wave workgroup per CU)."""
+    inp, ref, _ = references["fp16"]
+    out = vd.Buffer(SHAPE, vd.complex64)
+    Mixed3D(SHAPE, inp.var_type, run=8)(out, inp)
+    assert O.rel_l2(out.read(0), ref) <= 5e-4
 
 
 def test_fft3_mixed_two_pass_repeat_and_graph(references):

@@ -278,11 +278,11 @@ __device__ __forceinline__ void tile_stage2(const Dft16Ops& o, const unsigned sh
   }
 }
 
-// 16 tiles of a wave, pipelined; get(j, cur) fills the stage-1 inputs of tile j.  v[j][e]
-// receives X_j[r + 16 (4g + e)].  ypw: 2 x NT x 2 KB.
-template <int NT, class SIn, class Get>
-__device__ __forceinline__ void tiles16(const Dft16Ops& o, Get&& get, unsigned short* ypw, int r, int g,
-                                        cpx<float> (&v)[16][4]) {
+// 16 tiles of a wave, pipelined; get(j, cur) fills the stage-1 inputs of tile j and
+// put(j, zr, zi) takes its result, X_j[r + 16 (4g + e)] = (zr[e], zi[e]).
+// ypw: 2 x NT x 2 KB.
+template <int NT, class SIn, class Get, class Put>
+__device__ __forceinline__ void tiles16(const Dft16Ops& o, Get&& get, Put&& put, unsigned short* ypw, int r, int g) {
   constexpr int G = 16 / NT, BUF = NT * 4 * 256;
   static_assert(16 % NT == 0, "tile groups");
   const int part = g >> 1;
@@ -307,12 +307,51 @@ __device__ __forceinline__ void tiles16(const Dft16Ops& o, Get&& get, unsigned s
     f32x4_t zr[NT], zi[NT];
     tile_stage2<NT>(o, ypw + (j & 1) * BUF, r, g, zr, zi);
 #pragma unroll
-    for (int t = 0; t < NT; ++t)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) v[j * NT + t][e] = cpx<float>{zr[t][e], zi[t][e]};
+    for (int t = 0; t < NT; ++t) put(j * NT + t, zr[t], zi[t]);
     if (j + 1 < G) tile_twiddle_write<NT>(o, yr, yi, ypw + ((j + 1) & 1) * BUF, r, g);
     wave_sync_lds();
   }
+}
+
+// Tile results kept between the tile stages and the cross stage: FP32 (v32, 128 VGPRs for
+// 16 tiles) or fp16 pairs (v16, 64 VGPRs; one 11-bit rounding, the values pre-scaled by
+// an exact power of two so they stay in fp16 range).
+__device__ __forceinline__ unsigned pack_f16(float re, float im) {
+  const _Float16 a = (_Float16)re, b = (_Float16)im;
+  return (unsigned)__builtin_bit_cast(unsigned short, a) | ((unsigned)__builtin_bit_cast(unsigned short, b) << 16);
+}
+
+// cross stage on fp16 tile results: planes [g0][part][j / 8][r'][j % 8] (4 KB per wave),
+// the stored values are exact fp16 operands, so each output part is two products
+__device__ __forceinline__ void cross16_write16(const unsigned (&v)[16][4], int e0, unsigned short* xw, int r, int g) {
+  typedef unsigned u32x4_t __attribute__((ext_vector_type(4)));
+#pragma unroll
+  for (int pp = 0; pp < 2; ++pp) {
+    const unsigned sel = pp ? 0x07060302u : 0x05040100u;
+    u32x4_t w0, w1;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      w0[k] = __builtin_amdgcn_perm(v[2 * k + 1][e0], v[2 * k][e0], sel);
+      w1[k] = __builtin_amdgcn_perm(v[2 * k + 9][e0], v[2 * k + 8][e0], sel);
+    }
+    unsigned short* d = xw + ((g * 2 + pp) * 2) * 128 + r * 8;
+    *(u32x4_t*)(d) = w0;
+    *(u32x4_t*)(d + 128) = w1;
+  }
+}
+
+__device__ __forceinline__ void cross16_gemm16(const Dft16Ops& o, const unsigned short* xw, int g0, int r, int g,
+                                               f32x4_t& wr, f32x4_t& wi) {
+  const int h = g & 1, part = g >> 1;
+  const unsigned short* sp = xw + ((g0 * 2 + part) * 2 + h) * 128 + r * 8;
+  const f16x8_t a = __builtin_bit_cast(f16x8_t, *(const u16x8_t*)(sp));
+  f32x4_t x = {0.f, 0.f, 0.f, 0.f}, y = {0.f, 0.f, 0.f, 0.f};
+  x = mma_f16(a, o.hre_hi, x);
+  x = mma_f16(a, o.hre_lo, x);
+  y = mma_f16(a, o.him_hi, y);
+  y = mma_f16(a, o.him_lo, y);
+  wr = x;
+  wi = y;
 }
 
 // ---------------------------------------------------------------------------
@@ -457,18 +496,18 @@ __device__ __forceinline__ void cross16_chunk(const Dft16Ops& o, const cpx<float
 // transform of the 16 rows i1 = a1 + 16 b1 (tile256 per row), then the first axis-1
 // stage across those rows (cross16: DFT over b1, twiddle W256^(a1 c1)); the result
 // goes out as fp16 pairs scaled by 1/64 to the intermediate
-//   M[c1][k2 / 4][i0][a1][k2 % 4]
-// (each lane stores the four consecutive k2 it holds as 16 bytes; the four waves fill
-// 64 contiguous bytes).  Pass 2 (workgroup = c1 and four consecutive k2, one per wave)
-// loads its contiguous 64 KB block of M into LDS, runs the full axis-0 transform of
-// each (a1, k2) column (tile256 over i0) and the second axis-1 stage across a1
-// (cross16: DFT over a1), and writes X[k0][c1 + 16 d1][k2] as FP32 through a staging
-// tile, 32 contiguous bytes (four k2) per row.
+//   M[c1][k2 / P][i0][a1][k2 % P]        (P = 4 or 8)
+// through an LDS tile, in runs of 4 a1 x P k2 (16 P bytes).  Pass 2 (workgroup = c1
+// and P consecutive k2, one wave each) loads its contiguous 16 P KB block of M into
+// LDS, runs the full axis-0 transform of each (a1, k2) column and the second axis-1
+// stage across a1 (cross16: DFT over a1), and writes X[k0][c1 + 16 d1][k2] as FP32
+// through a staging tile, 8 P contiguous bytes (P consecutive k2) per row.
 // ---------------------------------------------------------------------------
 template <class C, class SIn>
 __device__ __forceinline__ void fft3_p1_body(const KParams& prm) {
-  constexpr int N = 256, PF = C::PF;
+  constexpr int N = 256, PF = C::PF, P = C::P;
   static_assert(C::BLOCK == 256, "four waves");
+  static_assert(P == 4 || P == 8, "k2 per intermediate run");
   constexpr int NT = C::NT;
   struct alignas(16) quad_t { cpx<_Float16> q[4]; };
   // per-wave double-buffered planes; after the tiles the same bytes hold the output
@@ -476,7 +515,8 @@ __device__ __forceinline__ void fft3_p1_body(const KParams& prm) {
   // padding after every four rows (conflict-free writes along c1)
   constexpr int YP = 4 * 2 * NT * 4 * N * 2, OST = 1088 * 16;
   __shared__ __attribute__((aligned(16))) unsigned char ypo[YP > OST ? YP : OST];
-  __shared__ __attribute__((aligned(16))) unsigned short xw[4][4096];
+  constexpr bool VH = C::VH;  // tile results held as fp16 pairs
+  __shared__ __attribute__((aligned(16))) unsigned short xw[4][VH ? 2048 : 4096];
   quad_t* ost = (quad_t*)ypo;
   const int tid = (int)threadIdx.x;
   const int lane = tid & 63;
@@ -489,11 +529,17 @@ __device__ __forceinline__ void fft3_p1_body(const KParams& prm) {
   const u32 b = blockIdx.x, xcd = b & 7, slot = b >> 3;
   const u32 ag = slot & 3, i0 = (slot >> 2) * 8 + xcd;
   const int a1 = (int)ag * 4 + wave;
-  const Dft16Ops ops = dft16_ops<C::DIR>(roots, r, g);
+  Dft16Ops ops = dft16_ops<C::DIR>(roots, r, g);
+  // fp16 tile results are scaled by 1/16 (in the stage twiddles), undone in the cross GEMM
+  constexpr float VS = VH ? 1.f / 16.f : 1.f;
+  if constexpr (VH)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) ops.tw[e] = cpx<float>{ops.tw[e].x * VS, ops.tw[e].y * VS};
   // the cross16 GEMM's output column c1 = r carries the twiddle W256^(a1 c1) and the scale
   cpx<float> t1 = roots[(a1 * r) & 255];
   if constexpr (C::DIR > 0) t1.y = -t1.y;
-  const Dft16Ops ops1 = dft16_ops<C::DIR>(roots, r, g, cpx<float>{t1.x * (float)C::OUT_SCALE, t1.y * (float)C::OUT_SCALE});
+  const float os = (float)C::OUT_SCALE / VS;
+  const Dft16Ops ops1 = dft16_ops<C::DIR>(roots, r, g, cpx<float>{t1.x * os, t1.y * os});
 
   const cpx<SIn>* rows = in + ((u64)i0 * N + (u32)a1) * N + r + 128 * h;  // row b1 at + 16 N b1
   cpx<SIn> ld[16][8];
@@ -502,6 +548,7 @@ __device__ __forceinline__ void fft3_p1_body(const KParams& prm) {
 #pragma unroll
     for (int i = 0; i < 8; ++i) ld[j][i] = rows[(u64)(16 * j) * N + 16 * i];
   cpx<float> v[16][4];
+  unsigned vh[16][4];
   tiles16<NT, SIn>(ops, [&](int j, cpx<SIn> (&cur)[8]) {
     if (j + PF < 16) {
 #pragma unroll
@@ -509,30 +556,60 @@ __device__ __forceinline__ void fft3_p1_body(const KParams& prm) {
     }
 #pragma unroll
     for (int i = 0; i < 8; ++i) cur[i] = ld[j][i];
-  }, (unsigned short*)ypo + wave * (2 * NT * 4 * N), r, g, v);
+  }, [&](int j, const f32x4_t& zr, const f32x4_t& zi) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      if constexpr (VH)
+        vh[j][e] = pack_f16(zr[e], zi[e]);
+      else
+        v[j][e] = cpx<float>{zr[e], zi[e]};
+    }
+  }, (unsigned short*)ypo + wave * (2 * NT * 4 * N), r, g);
   __syncthreads();  // the planes become the output staging
   unsigned short* xww = &xw[wave][0];
 #pragma unroll
   for (int e0 = 0; e0 < 4; ++e0) {
     f32x4_t wr[4], wi[4];
-    cross16_chunk(ops1, v, e0, xww, r, g, wr, wi);
-    // lane (c1 = r, g) holds k2 = 4 kb + e, kb = 4 (4 g0 + e0) + g: its 16 bytes go next to
-    // the other waves' (a1) in the staging tile [g0][c1][g][a1 % 4]
+    if constexpr (VH) {
+      cross16_write16(vh, e0, xww, r, g);
+      wave_sync_lds();
+#pragma unroll
+      for (int g0 = 0; g0 < 4; ++g0) cross16_gemm16(ops1, xww, g0, r, g, wr[g0], wi[g0]);
+      wave_sync_lds();
+    } else {
+      cross16_chunk(ops1, v, e0, xww, r, g, wr, wi);
+    }
+    // lane (c1 = r, g) holds k2 = 16 (4 g0 + e0) + 4g + e; its 16 bytes go next to the
+    // other waves' (a1) in the staging tile, one 16 P-byte row per (g0, c1, k2 / P)
 #pragma unroll
     for (int g0 = 0; g0 < 4; ++g0) {
       quad_t q;
 #pragma unroll
       for (int e = 0; e < 4; ++e) q.q[e] = ccvt<_Float16>(cpx<float>{wr[g0][e], wi[g0][e]});
-      const int row = (g0 * 16 + r) * 4 + g;
-      ost[row * 4 + (row >> 2) + wave] = q;
+      if constexpr (P == 4) {
+        const int row = (g0 * 16 + r) * 4 + g;
+        ost[row * 4 + (row >> 2) + wave] = q;
+      } else {
+        const int row = (g0 * 16 + r) * 2 + (g >> 1);
+        ost[row * 8 + (row >> 1) + 2 * wave + (g & 1)] = q;
+      }
     }
     __syncthreads();
 #pragma unroll
-    for (int it = 0; it < 4; ++it) {  // 256 runs of 64 bytes, four lanes each
-      const int qi = it * 256 + tid, run = qi >> 2, w = qi & 3;
-      const int g0 = run >> 6, c1 = (run >> 2) & 15, gg = run & 3;
-      const u32 kb = (u32)(4 * (4 * g0 + e0) + gg);
-      *(quad_t*)(mid + ((((u64)c1 * 64 + kb) * N + i0) * 16 + ag * 4 + (u32)w) * 4) = ost[qi + (qi >> 4)];
+    for (int it = 0; it < 4; ++it) {  // 1024 quads: runs of 4 a1 x P k2 (16 P bytes)
+      const int qi = it * 256 + tid;
+      if constexpr (P == 4) {
+        const int run = qi >> 2, w = qi & 3;
+        const int g0 = run >> 6, c1 = (run >> 2) & 15, gg = run & 3;
+        const u32 kb = (u32)(4 * (4 * g0 + e0) + gg);
+        *(quad_t*)(mid + ((((u64)c1 * 64 + kb) * N + i0) * 16 + ag * 4 + (u32)w) * 4) = ost[qi + (qi >> 4)];
+      } else {
+        const int row = qi >> 3, pos = qi & 7;
+        const int g0 = row >> 5, c1 = (row >> 1) & 15, gh = row & 1;
+        const u32 kb = (u32)(2 * (4 * g0 + e0) + gh);
+        *(quad_t*)(mid + ((((u64)c1 * 32 + kb) * N + i0) * 16 + ag * 4 + (u32)(pos >> 1)) * 8 + 4 * (pos & 1)) =
+            ost[qi + (qi >> 4)];
+      }
     }
     __syncthreads();
   }
@@ -541,12 +618,12 @@ __device__ __forceinline__ void fft3_p1_body(const KParams& prm) {
 template <class C>
 __device__ __forceinline__ void fft3_p2_body(const KParams& prm) {
   constexpr int N = 256;
-  constexpr int NT = C::NT;
-  static_assert(C::BLOCK == 256, "four waves");
-  // block tile: dword (a0, b0, a1, p) at a0 * 1024 + b0 * 64 + ((4 a1 + p) ^ (a0 + 16 (b0 >> 3))),
+  constexpr int NT = C::NT, P = C::P, KB = 256 / P, NTH = 64 * P;
+  static_assert(C::BLOCK == NTH && (P == 4 || P == 8), "one wave per k2");
+  // block tile: dword (a0, b0, a1, p) at a0 * 256 P + b0 * 16 P + ((P a1 + p) ^ (a0 + 16 (b0 >> 3))),
   // i0 = a0 + 16 b0: the 32 addresses of one stage-1 operand read fall in 32 distinct banks
-  __shared__ __attribute__((aligned(16))) unsigned blk[16384];
-  __shared__ __attribute__((aligned(16))) unsigned short yp[4][2 * NT * 4 * N];
+  __shared__ __attribute__((aligned(16))) unsigned blk[4096 * P];
+  __shared__ __attribute__((aligned(16))) unsigned short yp[P][2 * NT * 4 * N];
   const int tid = (int)threadIdx.x;
   const int lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -554,24 +631,29 @@ __device__ __forceinline__ void fft3_p2_body(const KParams& prm) {
   const cpx<_Float16>* mid = bufp<const cpx<_Float16>>(prm, 1);
   cpx<float>* out = bufp<cpx<float>>(prm, 0);
   const cpx<float>* roots = bufp<const cpx<float>>(prm, SLOT_ROOTS);
-  // consecutive workgroups of one XCD take consecutive k2 quads of one c1
+  // consecutive workgroups of one XCD take consecutive k2 blocks of one c1
   const u32 b = blockIdx.x, xcd = b & 7, slot = b >> 3;
-  const u32 kb = slot & 63, c1 = (slot >> 6) * 8 + xcd;
-  const Dft16Ops ops = dft16_ops<C::DIR>(roots, r, g);
-  const Dft16Ops ops1 = dft16_ops<C::DIR>(roots, r, g, cpx<float>{(float)C::OUT_SCALE, 0.f});
+  const u32 kb = slot % KB, c1 = (slot / KB) * 8 + xcd;
+  constexpr bool VH = C::VH;  // tile results held as fp16 pairs, scaled by 1/16
+  constexpr float VS = VH ? 1.f / 16.f : 1.f;
+  Dft16Ops ops = dft16_ops<C::DIR>(roots, r, g);
+  if constexpr (VH)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) ops.tw[e] = cpx<float>{ops.tw[e].x * VS, ops.tw[e].y * VS};
+  const Dft16Ops ops1 = dft16_ops<C::DIR>(roots, r, g, cpx<float>{(float)C::OUT_SCALE / VS, 0.f});
 
-  // ---- the block M[c1][kb][i0][a1][0..3] (16384 fp16 pairs) into LDS ----
+  // ---- the block M[c1][kb][i0][a1][0..P) (4096 P fp16 pairs) into LDS ----
   {
-    const uint4* src = (const uint4*)(mid + ((u64)c1 * 64 + kb) * 16384);
+    const uint4* src = (const uint4*)(mid + ((u64)c1 * KB + kb) * (4096 * P));
     uint4 q[16];
 #pragma unroll
-    for (int it = 0; it < 16; ++it) q[it] = src[it * 256 + tid];
+    for (int it = 0; it < 16; ++it) q[it] = src[it * NTH + tid];
 #pragma unroll
     for (int it = 0; it < 16; ++it) {
-      const int e4 = it * 256 + tid, i0 = e4 >> 4, a1 = e4 & 15;
+      const int e4 = it * NTH + tid, qq = e4 % (P / 4), a1 = (e4 / (P / 4)) & 15, i0 = e4 / (4 * P);
       const int a0 = i0 & 15, b0 = i0 >> 4, f = a0 + 16 * (b0 >> 3);
-      unsigned* d = blk + a0 * 1024 + b0 * 64;
-      const int o = 4 * a1;
+      unsigned* d = blk + a0 * (256 * P) + b0 * (16 * P);
+      const int o = P * a1 + 4 * qq;
       d[(o + 0) ^ f] = q[it].x;
       d[(o + 1) ^ f] = q[it].y;
       d[(o + 2) ^ f] = q[it].z;
@@ -579,36 +661,56 @@ __device__ __forceinline__ void fft3_p2_body(const KParams& prm) {
     }
   }
   __syncthreads();
-  // ---- axis 0: wave p = k2 % 4, tile j = a1, lane (r, g) reads i0 = r + 16 (8h + i) ----
+  // ---- axis 0: wave p = k2 % P, tile j = a1, lane (r, g) reads i0 = r + 16 (8h + i) ----
   const int p = wave;
   cpx<float> v[16][4];
+  unsigned vh[16][4];
   const int f = r + 16 * h;
   tiles16<NT, _Float16>(ops, [&](int j, cpx<_Float16> (&cur)[8]) {
 #pragma unroll
     for (int i = 0; i < 8; ++i)
-      cur[i] = __builtin_bit_cast(cpx<_Float16>, blk[r * 1024 + (8 * h + i) * 64 + ((4 * j + p) ^ f)]);
-  }, &yp[wave][0], r, g, v);
-  __syncthreads();  // blk is reused: cross16 planes (32 KB) + output staging (32 KB)
+      cur[i] = __builtin_bit_cast(cpx<_Float16>, blk[r * (256 * P) + (8 * h + i) * (16 * P) + ((P * j + p) ^ f)]);
+  }, [&](int j, const f32x4_t& zr, const f32x4_t& zi) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      if constexpr (VH)
+        vh[j][e] = pack_f16(zr[e], zi[e]);
+      else
+        v[j][e] = cpx<float>{zr[e], zi[e]};
+    }
+  }, &yp[wave][0], r, g);
+  __syncthreads();  // blk is reused: cross16 planes (8 KB per wave) + output staging
   unsigned short* xww = (unsigned short*)blk + wave * 4096;
-  // output staging [g0][k0 % 16][d1][p], p rotated by d1 / 4 (conflict-free writes along d1)
-  float2* stg = (float2*)(blk + 8192);
+  // output staging [g0][k0 % 16][d1][p] in 8-byte slots, p rotated by d1 / 4 (conflict-free
+  // writes along d1); one row (g0, k0 % 16, d1) = the P consecutive k2 of one output run
+  float2* stg = (float2*)(blk + 2048 * P);
+  const int rot = (r >> 2) * (P / 4);
 #pragma unroll
   for (int e0 = 0; e0 < 4; ++e0) {
     f32x4_t wr[4], wi[4];
-    cross16_chunk(ops1, v, e0, xww, r, g, wr, wi);
+    if constexpr (VH) {
+      cross16_write16(vh, e0, xww, r, g);
+      wave_sync_lds();
+#pragma unroll
+      for (int g0 = 0; g0 < 4; ++g0) cross16_gemm16(ops1, xww, g0, r, g, wr[g0], wi[g0]);
+      wave_sync_lds();
+    } else {
+      cross16_chunk(ops1, v, e0, xww, r, g, wr, wi);
+    }
 #pragma unroll
     for (int g0 = 0; g0 < 4; ++g0)
 #pragma unroll
       for (int e = 0; e < 4; ++e)  // lane (d1 = r, g): k0 = 4g + e + 16 (4 g0 + e0)
-        stg[(g0 * 16 + 4 * g + e) * 64 + 4 * r + ((p + (r >> 2)) & 3)] = float2{wr[g0][e], wi[g0][e]};
+        stg[((g0 * 16 + 4 * g + e) * 16 + r) * P + ((p + rot) & (P - 1))] = float2{wr[g0][e], wi[g0][e]};
     __syncthreads();
 #pragma unroll
-    for (int it = 0; it < 8; ++it) {  // 1024 runs of 32 bytes, two lanes each
-      const int qi = it * 256 + tid, row = qi >> 1, half = qi & 1;  // row = (g0, k0 % 16, d1)
+    for (int it = 0; it < 8; ++it) {  // 1024 runs of 8 P bytes, P / 2 lanes each
+      const int qi = it * NTH + tid, row = qi / (P / 2), quarter = qi % (P / 2);  // row = (g0, k0 % 16, d1)
       const int g0 = row >> 8, k0 = ((row >> 4) & 15) + 16 * (4 * g0 + e0), d1 = row & 15;
-      const float2* sr = stg + (row >> 4) * 64 + 4 * d1;
-      const float2 v0 = sr[(2 * half + (d1 >> 2)) & 3], v1 = sr[(2 * half + 1 + (d1 >> 2)) & 3];
-      *(float4*)(out + ((u64)k0 * 65536 + (c1 + 16 * (u32)d1) * 256 + 4 * kb + 2 * half)) = float4{v0.x, v0.y, v1.x, v1.y};
+      const float2* sr = stg + row * P;
+      const int rr = (d1 >> 2) * (P / 4);
+      const float2 v0 = sr[(2 * quarter + rr) & (P - 1)], v1 = sr[(2 * quarter + 1 + rr) & (P - 1)];
+      *(float4*)(out + ((u64)k0 * 65536 + (c1 + 16 * (u32)d1) * 256 + P * kb + 2 * quarter)) = float4{v0.x, v0.y, v1.x, v1.y};
     }
     __syncthreads();
   }

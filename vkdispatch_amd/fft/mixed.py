@@ -68,7 +68,8 @@ def _mfma_tu(cfg: str, body: str, block: int, symbol: str, waves_per_simd: Optio
 
 
 def make_cube_pass(which: int, in_type, inverse: bool = False, out_scale: float = 1.0, prefetch: int = 4,
-                   waves_per_simd: Optional[int] = 2, tiles: Optional[int] = None) -> CompiledKernel:
+                   waves_per_simd: Optional[int] = 2, tiles: Optional[int] = None, run: int = 4,
+                   half_tiles: bool = False) -> CompiledKernel:
     """One pass of the two-pass 256^3 transform (``fft3_p1_body`` / ``fft3_p2_body``):
     pass 1 reads the 16-bit volume (slot 1) and writes the fp16 intermediate (slot 0),
     pass 2 reads the intermediate (slot 1) and writes complex64 in natural order (slot 0).
@@ -83,18 +84,23 @@ def make_cube_pass(which: int, in_type, inverse: bool = False, out_scale: float 
         body = "vkx::fft3_p2_body<Cfg>(prm);"
     else:
         raise ValueError("pass is 1 or 2")
-    block, nwg = 256, 1024
+    if run not in (4, 8):
+        raise ValueError("run (k2 per intermediate run) is 4 or 8")
+    block, nwg = (256, 1024) if which == 1 else (64 * run, 16 * (256 // run))
     if tiles is None:
         tiles = 2 if which == 1 else 1  # tiles per plane exchange (LDS: 2 workgroups per CU)
     plan = KernelPlan(real="float", N=256, TT=block, B=1, S=1, OUTER=256 * 256, radices=(16, 16, 16), COLFAST=False,
                       WAVE_LOCAL=True, PAD_SHIFT=5, NP=0, LDS_ELEMS=0, NWG=nwg, NWG_INNER=1)
     cfg = "\n".join([
         "struct Cfg {",
-        f"  static constexpr int BLOCK = {block}, DIR = {1 if inverse else -1}, PF = {int(prefetch)}, NT = {int(tiles)};",
+        f"  static constexpr int BLOCK = {block}, DIR = {1 if inverse else -1}, PF = {int(prefetch)}, NT = {int(tiles)},",
+        f"                       P = {int(run)};",
+        f"  static constexpr bool VH = {'true' if half_tiles else 'false'};",
         f"  static constexpr double OUT_SCALE = {out_scale!r};",
         "};",
     ])
-    tag = f"p{which}_" + ("inv" if inverse else "fwd") + f"_{in_type.name}"
+    tag = f"p{which}_" + ("inv" if inverse else "fwd") + f"_{in_type.name}" + ("" if run == 4 else f"_r{run}") + \
+        ("_h" if half_tiles else "")
     src = _mfma_tu(cfg, body, block, kernel_symbol("mfma3d", plan, tag), waves_per_simd)
     args = [Arg("output", "buffer", out_type, 0), Arg("input", "buffer", in_type, 1)]
     return CompiledKernel(src, plan, args, None, None, name=f"mfma3d_p{which}", roots_n=256)
@@ -103,11 +109,12 @@ def make_cube_pass(which: int, in_type, inverse: bool = False, out_scale: float 
 class Mixed3D:
     """Plan for the 3-D mixed-precision transform of one 256 x 256 x 256 volume, complex64
     result in natural order.  ``passes=2`` (default): the two-pass form, 16-bit input ->
-    fp16 intermediate -> complex64 (three radix-16 MFMA stages per pass).  ``passes=3``:
-    three transposed-write 256-point passes with 16-bit storage in between."""
+    fp16 intermediate -> complex64 (three radix-16 MFMA stages per pass); ``run`` = 4 or 8
+    consecutive k2 per intermediate run (8: one 8-wave pass-2 workgroup per CU).
+    ``passes=3``: three transposed-write 256-point passes with 16-bit storage in between."""
 
     def __init__(self, shape, in_type=dt.complex_bf16, inverse: bool = False, normalize: bool = True,
-                 block: int = 256, T16: Optional[int] = None, passes: int = 2):
+                 block: int = 256, T16: Optional[int] = None, passes: int = 2, run: int = 4):
         shape = tuple(int(s) for s in shape)
         if len(shape) != 3 or any(s != 256 for s in shape):
             raise ValueError("fft3_mixed handles 256 x 256 x 256 volumes")
@@ -123,8 +130,10 @@ class Mixed3D:
         mid = dt.complex32
         if passes == 2:
             # 1/64 after pass 1 (three radix-16 stages) keeps N(0,1)-scale data near unit size in fp16
-            self.passes = [make_cube_pass(1, in_type, inverse, 1.0 / 64.0),
-                           make_cube_pass(2, mid, inverse, 64.0 * norm)]
+            # pass 1 holds its tile results as fp16 (one extra 11-bit rounding, fewer
+            # registers and a two-product cross stage); pass 2 keeps them in FP32
+            self.passes = [make_cube_pass(1, in_type, inverse, 1.0 / 64.0, run=run, half_tiles=True),
+                           make_cube_pass(2, mid, inverse, 64.0 * norm, run=run)]
         else:
             nb = 256 * 256
             s_mid = 1.0 / 16.0
