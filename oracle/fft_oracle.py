@@ -17,7 +17,11 @@ Reference behaviour restated here:
     ``kernel_inner_only`` broadcasts K over the outer batch
     (global_memory_iterators.py:26-27);
   * convolve2D / convolve2DR = fft(last) -> convolve(ndim-2) -> ifft(last)
-    (vkdispatch/fft/functions.py:549-646).
+    (vkdispatch/fft/functions.py:549-646);
+  * vd.vkfft.convolve2D(x, K) = ifft2(fft2(x) * K) with K = fft2(k) prepared by
+    transpose_kernel2D, no conjugate unless ``conjugate_kernel``, unnormalised
+    unless ``normalize`` (vkdispatch/vkfft/vkfft_dispatcher.py:109-167, checked by
+    tests/test_vkfft_conv.py:10-15,62-66).
 
 The DFT core is a mixed-radix Stockham decimation-in-time in float64, written
 for this oracle (it does not call numpy.fft); numpy.fft is used only by the
@@ -32,7 +36,7 @@ import numpy as np
 __all__ = [
     "prime_factors", "dft", "idft", "dft_matrix", "bluestein_dft", "fft_axis", "fftn_axes", "rfft_rows",
     "irfft_rows", "pack_real", "unpack_real", "convolve_axis", "convolve2D", "convolve2DR",
-    "correlate_bruteforce", "rel_l2",
+    "correlate_bruteforce", "circular_convolve_bruteforce", "vkfft_convolve2D", "rel_l2",
 ]
 
 
@@ -237,6 +241,27 @@ def convolve2DR(x: np.ndarray, K_half: np.ndarray, normalize: bool = True, kerne
     X = rfft_rows(x)
     X = convolve_axis(X, K_half, x.ndim - 2, normalize, kernel_inner_only)
     return irfft_rows(X, n, normalize)
+
+
+def vkfft_convolve2D(x: np.ndarray, K: np.ndarray, normalize: bool = False, conjugate_kernel: bool = False):
+    """vd.vkfft.convolve2D(buf, K) over the last two axes of x; K is one spectrum per
+    image or one broadcast over the leading axis (vkfft_dispatcher.py:137-167)."""
+    X = fftn_axes(x, (x.ndim - 1, x.ndim - 2))
+    Kc = np.conj(K) if conjugate_kernel else np.asarray(K, dtype=np.complex128)
+    return fftn_axes(X * Kc, (x.ndim - 1, x.ndim - 2), inverse=True, normalize=normalize)
+
+
+def circular_convolve_bruteforce(x: np.ndarray, k: np.ndarray) -> np.ndarray:
+    """y[n] = sum_m x[m] k[(n - m) mod N] over the last two axes (independent pin of
+    ifft2(fft2(x) * fft2(k)), normalised); x may carry one leading batch axis."""
+    x = np.asarray(x, dtype=np.complex128)
+    k = np.asarray(k, dtype=np.complex128)
+    H, W = x.shape[-2:]
+    y = np.zeros(np.broadcast_shapes(x.shape, k.shape), dtype=np.complex128)
+    for m0 in range(H):
+        for m1 in range(W):
+            y += x[..., m0:m0 + 1, m1:m1 + 1] * np.roll(np.roll(k, m0, axis=-2), m1, axis=-1)
+    return y
 
 
 def correlate_bruteforce(x: np.ndarray, k: np.ndarray) -> np.ndarray:

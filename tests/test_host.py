@@ -224,3 +224,18 @@ def test_pair_conv_kernels_do_not_spill():
             for k in fused.fused_plan(batch, n, n, norm, mode).kernels():
                 info = k.code_object_info()
                 assert info["private_bytes"] == 0, (k.name, info)
+
+
+def test_map_resolves_postponed_annotations():
+    """Hook parameters annotated under ``from __future__ import annotations`` (strings)
+    resolve to the same buffer types."""
+    src = ("from __future__ import annotations\n"
+           "import vkdispatch_amd as vd\n"
+           "import vkdispatch_amd.codegen as vc\n"
+           "def hook(buf: vc.Buffer[vd.complex64], s: vc.Var[vd.float32]):\n"
+           "    pass\n")
+    ns = {}
+    exec(compile(src, "<postponed>", "exec"), ns)
+    m = vd.map(ns["hook"])
+    import vkdispatch_amd.codegen as vc
+    assert m.buffer_types == [vc.Buffer[vd.complex64], vc.Var[vd.float32]]

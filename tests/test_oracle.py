@@ -147,6 +147,27 @@ def test_convolve2D_and_2DR():
     assert np.allclose(yr, np.fft.ifft2(np.fft.fft2(xr) * np.conj(np.fft.fft2(kr))).real)
 
 
+def test_vkfft_convolve2D_is_circular_convolution():
+    """vd.vkfft.convolve2D semantics: ifft2(fft2(x) * fft2(k)) is circular convolution
+    (the reference test's numpy formula, tests/test_vkfft_conv.py:10-15); the
+    conjugated kernel gives cross-correlation; no 1/(HW) without normalize."""
+    x = crandn(3, 5, 6)
+    k = crandn(3, 5, 6)
+    K = np.fft.fft2(k)
+    y = O.vkfft_convolve2D(x, K, normalize=True)
+    assert np.allclose(y, O.circular_convolve_bruteforce(x, k))
+    assert np.allclose(y, np.fft.ifft2(np.fft.fft2(x) * K))
+    assert np.allclose(O.vkfft_convolve2D(x, K, normalize=False), 30 * y)
+    k1 = crandn(5, 6)
+    y1 = O.vkfft_convolve2D(x, np.fft.fft2(k1), normalize=True)          # one kernel, broadcast
+    assert np.allclose(y1, O.circular_convolve_bruteforce(x, k1))
+    yc = O.vkfft_convolve2D(x, np.fft.fft2(k1), normalize=True, conjugate_kernel=True)
+    assert np.allclose(yc, O.correlate_bruteforce(x, k1))
+    d = np.zeros((5, 6), complex)
+    d[2, 3] = 1                                                            # impulse -> shift
+    assert np.allclose(O.vkfft_convolve2D(x, np.fft.fft2(d), normalize=True), np.roll(x, (2, 3), axis=(1, 2)))
+
+
 def test_plausible_mistakes_are_caught():
     """The checks above distinguish the oracle from its likeliest wrong variants."""
     n = 12

@@ -26,5 +26,6 @@ from .graph import CommandGraph, global_graph
 from .runtime import Event, set_stream, stream_scope, synchronize
 from .shader_map import MappingFunction, map
 from . import fft
+from . import vkfft
 
 __version__ = "0.1.0"

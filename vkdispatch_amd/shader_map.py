@@ -42,10 +42,16 @@ def map(func: Callable = None, return_type=None, input_types: List = None):  # n
         return_type = sig.return_annotation
     if input_types is None:
         input_types = []
+        hints = None
         for param in sig.parameters.values():
             ann = param.annotation
             if ann is inspect.Parameter.empty:
                 raise ValueError("All parameters must be annotated")
+            if isinstance(ann, str):  # postponed evaluation (from __future__ import annotations)
+                if hints is None:
+                    import typing
+                    hints = typing.get_type_hints(func)
+                ann = hints.get(param.name, ann)
             if not hasattr(ann, "__args__"):
                 raise TypeError(f"Argument '{param.name}: {ann}' must have a type annotation")
             input_types.append(ann)
