@@ -32,6 +32,7 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+MFMA_PEAK_FLOPS = 2.5e15  # MI355X dense bf16 / fp16 matrix peak
 
 
 def conv_inputs(batch=64, n=512, seed=0):
@@ -313,6 +314,8 @@ def bench_mixed3d(steps, warmup, args):
                       "rel_l2_vs_fp64_quantized": O.rel_l2(r, np.fft.fftn(xq.astype(np.complex128))),
                       "rel_l2_vs_fp64_unquantized": O.rel_l2(r, ref_unq),
                       "passes": plan.npasses, "bytes": z.size * (4 + 8 * (plan.npasses - 1) + 8),
+                      "mfma_tflops": plan.mfma_count() * 16384 / (ms * 1e-3) / 1e12,
+                      "mfma_utilisation": plan.mfma_count() * 16384 / (ms * 1e-3) / MFMA_PEAK_FLOPS,
                       "speedup_vs_fp32": ms32 / ms}
         del inp, out, plan
     return {"metric": "3-D C2C 256^3: BF16-MFMA split-precision vs FP32 speedup at rel-L2<=1e-3",
@@ -359,6 +362,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="conv2d", choices=["conv2d", "c2c4096", "ragged", "mixed3d", "copy"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-mixed", action="store_true", help="conv2d: skip the BF16-MFMA 256^3 comparison")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "vkdispatch_amd benchmarks one GPU per process; multi-GPU sharding is "
@@ -366,6 +370,16 @@ def main():
         sys.exit(2)
     if args.workload == "conv2d":
         out = bench_conv(args.steps, args.warmup, args)
+        if not args.no_mixed:
+            # the metric's second half: BF16-MFMA 256^3 transform vs the FP32 path
+            m = bench_mixed3d(args.steps, args.warmup, args)
+            d = m["detail"]
+            out["bf16_mfma_vs_fp32"] = {
+                "value": m["value"], "unit": "x", "workload": "3-D C2C 256^3, bf16 storage, fp32 output",
+                "ms_bf16": d["bf16"]["ms"], "ms_fp32": d["fp32"]["ms"],
+                "rel_l2_vs_fp64_quantized": d["bf16"]["rel_l2_vs_fp64_quantized"],
+                "mfma_utilisation": d["bf16"]["mfma_utilisation"],
+                "fp16_storage_speedup": d["fp16"]["speedup_vs_fp32"]}
     elif args.workload == "ragged":
         out = bench_ragged(args.steps, args.warmup, args)
     elif args.workload == "mixed3d":

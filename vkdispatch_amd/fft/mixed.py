@@ -147,6 +147,17 @@ class Mixed3D:
     def kernels(self):
         return list(self.passes)
 
+    def mfma_count(self) -> int:
+        """v_mfma_f32_16x16x32 instructions (16384 flops each) per transform.  A 256-point
+        tile costs 4 products for the stage reading 16-bit storage (exact operand: hi and
+        lo parts of the DFT matrix, real and imaginary output) plus 6 for the stage on
+        split FP32 data; a cross stage tile costs 6 (FP32 tile results) or 4 (fp16)."""
+        tile = 4 + 6
+        if self.npasses == 3:
+            return 3 * 65536 * tile
+        cubes = 4096  # 16 x 16 x 16 blocks per pass, 16 tiles + 16 cross tiles each
+        return cubes * (16 * tile + 16 * 4) + cubes * (16 * tile + 16 * 6)
+
     def scratch(self) -> Tuple[Buffer, ...]:
         if self._scratch is None:
             n = 1 if self.npasses == 2 else 2
