@@ -366,7 +366,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "vkdispatch_amd benchmarks one GPU per process; multi-GPU sharding is "
-                                   "described in DESIGN.md and not run", "n_gpus": args.gpus}))
+                                   "a design note only and is not run", "n_gpus": args.gpus}))
         sys.exit(2)
     if args.workload == "conv2d":
         out = bench_conv(args.steps, args.warmup, args)

@@ -1,21 +1,25 @@
-// vkx_mfma.h -- 256-point FFT passes whose two radix-16 stages are complex
-// GEMMs on the gfx950 matrix cores (v_mfma_f32_16x16x32_bf16).
+// vkx_mfma.h -- 256-point FFT stages as complex GEMMs on the gfx950 matrix cores
+// (v_mfma_f32_16x16x32_bf16 / _f16), for 16-bit (bf16 or fp16) storage.
 //
 // A length-256 transform is viewed as X[n1][n2] = x[n1 + 16 n2]:
 //   stage 1  Y = X . F16            (DFT over n2, one 16x16x16 complex GEMM)
 //   twiddle  Y[n1][k2] *= W256^(n1 k2)
 //   stage 2  Z = F16 . Y            (DFT over n1)
 //   output   X[k2 + 16 k1] = Z[k1][k2]
-// Each complex GEMM is one real 16x32 by 32x16 product per output part, using
-// the 2x2 real embedding ([Xr | Xi] . [[Fr, Fi], [-Fi, Fr]]).  Operands are
-// split into bf16 hi + lo parts and three products (hi.hi + lo.hi + hi.lo) are
-// accumulated in FP32, so the arithmetic keeps ~16 significant bits.  F16 is
-// symmetric, so the same per-lane constant operands serve both stages.
+// Each complex GEMM is one real 16x32 by 32x16 product per output part, using the
+// 2x2 real embedding ([Xr | Xi] . [[Fr, Fi], [-Fi, Fr]]).  The DFT matrix is held as
+// hi + lo parts; FP32 data operands are split the same way and three products
+// (hi.hi + lo.hi + hi.lo) accumulate in FP32 (~16 significant bits); data read from
+// 16-bit storage is exact on the MFMA of its own type and needs two products.  F16
+// is symmetric, so the same per-lane constant operands serve both stages.
 //
 // MFMA 16x16x32 lane layout (checked on the device): A[r][k] is element k%8 of
 // lane r + 16*(k/8); B[k][c] is element k%8 of lane c + 16*(k/8); the result
 // D[r][c] is element r%4 of lane c + 16*(r/4).
 //
+// Contents: the per-wave tile machinery (tile256x, tiles16, cross16_*), the
+// transposed-write 256-point pass (fft256_mfma_tw_body, three passes per 256^3
+// volume) and the two-pass 256^3 transform (fft3_p1_body, fft3_p2_body).
 #pragma once
 #include "vkx_fft.h"
 
@@ -39,10 +43,6 @@ __device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
 }
 
 typedef unsigned short u16x8_t __attribute__((ext_vector_type(8)));
-typedef unsigned short u16x4_t __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ unsigned short bf16_bits(__bf16 v) { return __builtin_bit_cast(unsigned short, v); }
-
 typedef float f32x2_t __attribute__((ext_vector_type(2)));
 typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
 
