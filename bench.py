@@ -323,6 +323,58 @@ def bench_mixed3d(steps, warmup, args):
             "detail": res}
 
 
+def bench_autograd(steps, warmup, args):
+    """Differentiable convolution, 32 x 1024^2 float32 with a learnable 1024^2 kernel:
+    forward + loss 0.5||y - t||^2 + backward (dx, dk) per step, against the same step
+    written with torch.fft (rocFFT) autograd."""
+    import torch
+    from vkdispatch_amd.fft.autograd import fft_convolve2d
+    B, H, W = 32, 1024, 1024
+    rng = np.random.default_rng(0)
+    dev = "cuda"
+    x = torch.tensor(rng.standard_normal((B, H, W), dtype=np.float32), device=dev, requires_grad=True)
+    k = torch.tensor((0.01 * rng.standard_normal((H, W))).astype(np.float32), device=dev, requires_grad=True)
+    t = torch.tensor(rng.standard_normal((B, H, W), dtype=np.float32), device=dev)
+
+    def ours():
+        y = fft_convolve2d(x, k)
+        return 0.5 * ((y - t) ** 2).sum()
+
+    def vendor():
+        y = torch.fft.irfft2(torch.fft.rfft2(x) * torch.conj(torch.fft.rfft2(k)), s=(H, W))
+        return 0.5 * ((y - t) ** 2).sum()
+
+    def timed(loss_fn):
+        for _ in range(warmup):
+            x.grad = k.grad = None
+            loss_fn().backward()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            x.grad = k.grad = None
+            loss_fn().backward()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    ms = timed(ours)
+    dx, dk = x.grad.detach().clone(), k.grad.detach().clone()
+    ms_vendor = timed(vendor)
+    dx_v, dk_v = x.grad.detach(), k.grad.detach()
+    rel = lambda a, b: float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))  # noqa: E731
+    img = B * H * W * 4
+    # FFT path bytes per step: forward reads x and k, writes y; backward reads g and x,
+    # writes dx and dk (the loss's own elementwise kernels are not counted)
+    alg = (img + H * W * 4 + img) + (2 * img + img + H * W * 4)
+    return {"metric": "differentiable 2-D FFT convolution, forward + backward per step", "value": round(ms, 4),
+            "unit": "ms", "higher_is_better": False, "steps": steps, "warmup": warmup,
+            "config": {"workload": "32 x 1024^2 float32 images, learnable 1024^2 kernel, loss 0.5||y - t||^2"},
+            "fft_path_bytes_per_step": alg, "fft_path_gbs_lower_bound": round(alg / (ms * 1e-3) / 1e9, 1),
+            "torch_fft_autograd_ms": round(ms_vendor, 4), "speedup_vs_torch_fft": round(ms_vendor / ms, 3),
+            "rel_l2_dx_vs_torch_fft": rel(dx, dx_v), "rel_l2_dk_vs_torch_fft": rel(dk, dk_v)}
+
+
 def bench_copy(steps, warmup, args):
     """Plain device-to-device copy of 128 MiB (calibration of the HBM byte counters)."""
     import vkdispatch_amd as vd
@@ -360,7 +412,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="conv2d", choices=["conv2d", "c2c4096", "ragged", "mixed3d", "copy"])
+    ap.add_argument("--workload", default="conv2d", choices=["conv2d", "c2c4096", "ragged", "mixed3d", "autograd", "copy"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-mixed", action="store_true", help="conv2d: skip the BF16-MFMA 256^3 comparison")
     args = ap.parse_args()
@@ -384,6 +436,8 @@ def main():
         out = bench_ragged(args.steps, args.warmup, args)
     elif args.workload == "mixed3d":
         out = bench_mixed3d(args.steps, args.warmup, args)
+    elif args.workload == "autograd":
+        out = bench_autograd(args.steps, args.warmup, args)
     elif args.workload == "copy":
         out = bench_copy(args.steps, args.warmup, args)
     else:
