@@ -222,7 +222,9 @@ def bench_c2c(steps, warmup, args):
 
 
 def bench_ragged(steps, warmup, args):
-    """4096 ragged C2C transforms (60% pow2, 30% 7-smooth, 10% primes), one persistent launch."""
+    """4096 ragged C2C transforms (60% pow2, 30% 7-smooth, 10% primes): the persistent
+    ragged kernel, one launch per tile level, plan picked by the autotuner, replayed as
+    a captured graph."""
     import vkdispatch_amd as vd
     from vkdispatch_amd.fft import ragged as R
     import oracle as O
@@ -232,7 +234,7 @@ def bench_ragged(steps, warmup, args):
     buf = vd.Buffer((packed.size,), vd.complex64)
     tuned = R.tune(sizes, buf, reps=3)
     best = tuned["best"]
-    plan = R.RaggedFFT(sizes, block=best["block"], max_radix=best["max_radix"])
+    plan = R.RaggedFFT(sizes, block=best["block"], max_radix=best["max_radix"], tile_min=best["tile_min"])
     g = vd.CommandGraph()
     g.record_callable(plan.bind(buf, src))
     g.capture()

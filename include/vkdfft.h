@@ -156,13 +156,18 @@ int vkx_xcc_map(int* out, int nwg);
 
 /* ---- ragged batches ------------------------------------------------------------ */
 /* No counterpart in the reference (its FFTs are one kernel per shape, one length
- * per launch).  Runs `ntasks` 1-D C2C transform tasks of mixed lengths in one
- * persistent launch: `tasks` is a device array of 112-byte task records (layout in
- * vkdispatch_amd/fft/ragged.py, RTASK_DTYPE); `counter` a device int used as the
- * task queue (reset by the call); `scratch` holds `scratch_per_wg` complex64
- * elements per workgroup of `grid`; `block` in {64, 128, ..., 512}. */
-int vkx_ragged_exec(uint64_t tasks, int ntasks, uint64_t counter, uint64_t scratch, long long scratch_per_wg,
-                    int grid, int block, uint64_t stream);
+ * per launch).  Runs `ntasks` work items of 1-D C2C transforms of mixed lengths in
+ * one persistent launch: `tasks` is a device array of 128-byte item records (layout
+ * in vkdispatch_amd/fft/ragged.py, RTASK_DTYPE) in queue order, where an item only
+ * waits for items before it; `counters` holds `ncounters` >= 2 device ints, all
+ * zeroed by the call (queue position, abort flag, per-transform done counters);
+ * `scratch` holds `scratch_per_wg` complex64 elements per workgroup of `grid`;
+ * `block` in {64, 128, ..., 512}.  A waiting item polls its counter at most
+ * `spin_limit` times (<= 0: 2^20), then sets the abort flag (counters[1] = 1) and
+ * every waiting workgroup stops waiting, so the launch always ends; a result with
+ * the flag set is invalid. */
+int vkx_ragged_exec(uint64_t tasks, int ntasks, uint64_t counters, int ncounters, uint64_t scratch,
+                    long long scratch_per_wg, int grid, int block, int spin_limit, uint64_t stream);
 /* Bluestein kernel b[j] = exp(+i pi j^2 / n) for |j| < n (indices mod m), 0 elsewhere,
  * complex64, phases in FP64 with j^2 reduced mod 2n in integers; m >= 2n-1. */
 int vkx_bluestein_b(uint64_t dst, int n, int m, uint64_t stream);

@@ -91,6 +91,32 @@ def test_large_prime_needs_bluestein():
         P.radix_plan(2 * 37)
 
 
+@pytest.mark.parametrize("L", [4096, 4200, 8192, 59049, 60060, 65536, 131072, 120120, 2 * 3 * 5 * 7 * 11 * 13 * 16,
+                               1 << 18, 1 << 20, 4099 * 2, 19 * 4096])
+def test_four_step_split(L):
+    """Lengths above one LDS buffer split into n1 * n2 with both factors <= 512 (so a
+    tile holds >= 8 padded columns or rows) and at most 12 radices in all."""
+    from vkdispatch_amd.fft import ragged as R
+    s = R.four_step_split(L)
+    if L <= R.LDS_PINGPONG:
+        assert s is None
+        return
+    if L > R.FOUR_STEP_MAX ** 2 or L in (4099 * 2, 19 * 4096):  # too long / a prime above 13: no split
+        assert s is None
+        return
+    assert s is not None
+    r1, r2 = s
+    n1, n2 = int(np.prod(r1)), int(np.prod(r2))
+    assert n1 * n2 == L and max(n1, n2) <= R.FOUR_STEP_MAX and len(r1) + len(r2) <= 12
+    assert all(r in (2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 16) for r in r1 + r2)
+    t = np.zeros(1, R.RTASK_DTYPE)[0]
+    R._encode_passes(t, r1 + r2, s, R.K_ROW, R.CONJ)
+    v = int(t["npass"])
+    assert v & 0xff == len(r1) + len(r2) and (v >> 8) & 0xff == len(r1) and (v >> 16) & 0xf == R.K_ROW
+    assert (v >> 20) & 1 == 1
+    assert list(t["radix"][:len(r1) + len(r2)]) == r1 + r2
+
+
 # ---------------------------------------------------------------------------
 # precision rules (ports of the reference's fake-buffer tests,
 # tests/test_fft_mixed_precision.py:270-320)

@@ -56,9 +56,11 @@ def test_ragged_batch_small():
     buf = vd.asbuffer(packed)
     plan = R.RaggedFFT(sizes)
     plan(buf)
+    assert not plan.aborted()
     _check(sizes, parts, buf.read(0))
     inv = R.RaggedFFT(sizes, inverse=True)
     inv(buf)
+    assert not inv.aborted()
     assert O.rel_l2(buf.read(0), packed) < 1e-5
 
 
@@ -87,5 +89,43 @@ def test_ragged_4096_mixed_sizes_every_transform():
     buf = vd.asbuffer(packed)
     plan = R.RaggedFFT(sizes)
     plan(buf)
+    assert not plan.aborted()
     worst = _check(sizes, parts, buf.read(0))
     assert worst < 1e-5
+
+
+@pytest.mark.parametrize("four_step,sync", [(False, "launch"), (True, "launch"), (True, "counter")])
+def test_ragged_long_and_bluestein_both_paths(four_step, sync):
+    """Long smooth lengths and Bluestein primes through the tiled items (one launch per
+    level, or one launch with counter waits) and through whole-transform items,
+    forward and inverse."""
+    sizes = [8192, 65536, 4099, 59049, 12288, 65521, 100, 5003, 131072, 60060]
+    parts, packed = _packed(sizes, 9)
+    src = vd.asbuffer(packed)
+    dst = vd.Buffer((packed.size,), vd.complex64)
+    plan = R.RaggedFFT(sizes, four_step=four_step, max_radix=8 if four_step else 16, sync=sync, tile_min=4097)
+    plan(dst, src)
+    assert not plan.aborted()
+    _check(sizes, parts, dst.read(0))
+    inv = R.RaggedFFT(sizes, inverse=True, four_step=four_step, sync=sync, tile_min=4097)
+    inv(dst)
+    assert not inv.aborted()
+    assert O.rel_l2(dst.read(0), packed) < 1e-5
+
+
+def test_ragged_wait_gives_up_and_the_launch_ends():
+    """An item whose wait target is never reached (set from the host) polls its
+    counter spin_limit times, sets the abort flag and the launch still ends."""
+    sizes = [8192, 64]
+    _, packed = _packed(sizes, 2)
+    buf = vd.asbuffer(packed)
+    plan = R.RaggedFFT(sizes, spin_limit=1000, sync="counter", tile_min=4097)
+    waits = np.nonzero(plan._items["dep"] >= 0)[0]
+    assert waits.size > 0
+    plan._items["target"][waits[0]] += 1000
+    plan(buf)
+    assert plan.aborted()
+    ok = R.RaggedFFT(sizes, spin_limit=1000, sync="counter", tile_min=4097)
+    buf.write(packed)
+    ok(buf)
+    assert not ok.aborted()

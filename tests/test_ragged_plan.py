@@ -1,0 +1,193 @@
+"""CPU checks of the ragged work-item queue (vkdispatch_amd/fft/ragged.py): each item
+of a planned batch is executed in NumPy FP64 with the semantics the ragged kernel
+gives its kind (csrc/ragged.hip: column tile, row tile, the Bluestein levels), with
+every index checked against its region, every wait checked against the counters
+raised before it, and the result compared with numpy.fft.  No device is used."""
+import numpy as np
+import pytest
+
+from vkdispatch_amd.fft import ragged as R
+
+
+class _FakeTables:
+    """Stands in for the device tables: addresses are never dereferenced here."""
+
+    def twiddles(self, n, radices):
+        return 1
+
+    def twiddles_split(self, split):
+        return 2
+
+    def chirp_table(self, n):
+        return 3
+
+    def chirp_spectrum(self, n, m, radices):
+        return 4
+
+
+def _chirp(n, inverse):
+    j = np.arange(n, dtype=np.int64)
+    w = np.exp(-1j * np.pi * ((j * j) % (2 * n)) / n)
+    return np.conj(w) if inverse else w
+
+
+def _kernel_spectrum(n, m):
+    b = np.zeros(m, np.complex128)
+    j = np.arange(m, dtype=np.int64)
+    jj = np.where(j < n, j, np.where(j > m - n, m - j, -1))
+    ok = jj >= 0
+    b[ok] = np.exp(1j * np.pi * ((jj[ok] * jj[ok]) % (2 * n)) / n)
+    return np.fft.fft(b)
+
+
+def emulate(q, sizes, x, inverse, normalize=True):
+    sizes = list(sizes)
+    offsets = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(int)
+    bufs = {R.SRC_IN: x.astype(np.complex128).copy(), R.SRC_OUT: np.zeros(x.size, np.complex128),
+            R.SRC_SCR: np.full(max(1, q.scratch_elems), np.nan + 0j)}
+    size_at = dict(zip(offsets.tolist(), sizes))
+    done = {}
+    seg_of = np.zeros(len(q.items), int)
+    for s_i, (a, b) in enumerate(q.segments):
+        seg_of[a:b] = s_i
+    wseg = np.full(max(1, q.scratch_elems), -1)  # launch segment that wrote each scratch element
+    for k, t in enumerate(q.items):
+        v = int(t["npass"])
+        npass, np1, kind, cj, lw = v & 0xFF, (v >> 8) & 0xFF, (v >> 16) & 0xF, (v >> 20) & 1, (v >> 21) & 0xF
+        n, m = int(t["n"]), int(t["m"])
+        (sb, so), (db, do) = q.src[k], q.dst[k]
+        src, dst = bufs[sb], bufs[db]
+        if t["dep"] >= 0:
+            assert done.get(int(t["dep"]), 0) == int(t["target"]), f"item {k}: wait not satisfied"
+        if kind == R.K_WHOLE:
+            seg = src[so:so + n]
+            res = (np.fft.ifft(seg) * n if inverse else np.fft.fft(seg)) * float(t["scale"])
+            dst[do:do + n] = res
+            continue
+        L = m if m else n
+        rads = [int(r) for r in t["radix"][:npass]]
+        n1 = int(np.prod(rads[:np1]))
+        n2 = L // n1
+        assert n1 * n2 == L and int(np.prod(rads)) == L
+        # region limits: the user buffers hold n points at the transform's offset, scratch L
+        src_len = n if sb != R.SRC_SCR else L
+        dst_len = n if db != R.SRC_SCR else L
+        if sb != R.SRC_SCR:
+            assert size_at[so] == n
+        cnt, t0 = int(t["count"]), int(t["tile0"])
+        assert cnt >= 1
+        seg = seg_of[k]
+
+        def read_scr(ix):  # scratch written by an earlier launch, or by a counter-ordered item
+            if sb == R.SRC_SCR:
+                assert (wseg[so + ix] >= 0).all() and (t["dep"] >= 0 or (wseg[so + ix] < seg).all())
+            return src[so + ix]
+
+        def note_write(ix):
+            if db == R.SRC_SCR:
+                wseg[do + ix] = seg
+        if kind in (R.K_COL, R.K_BLUE_COL):
+            # tiles sit in LDS at pitch n1 + 1: one buffer, or two for a pass list of <= 2
+            assert t0 + cnt <= n2 and cnt <= (1 << lw)
+            assert (1 << lw) * (n1 + 1) <= R.LDS_TILE * (2 if np1 <= 2 else 1)
+            j2 = t0 + np.arange(cnt)
+            idx = np.arange(n1)[:, None] * n2 + j2[None, :]
+            if kind == R.K_COL:
+                assert idx.max() < src_len
+                col = read_scr(idx)
+                if cj:
+                    col = np.conj(col)
+            else:
+                w = _chirp(n, inverse)
+                col = np.zeros(idx.shape, np.complex128)
+                ok = idx < n
+                col[ok] = src[so + idx[ok]] * w[idx[ok]]
+            f = np.fft.fft(col, axis=0)
+            k1 = np.arange(n1)[:, None]
+            oidx = k1 * n2 + j2[None, :]
+            assert oidx.max() < dst_len
+            dst[do + oidx] = f * np.exp(-2j * np.pi * ((k1 * j2[None, :]) % L) / L)
+            note_write(oidx)
+        else:
+            np2 = npass - np1
+            assert t0 + cnt <= n1 and cnt <= (1 << lw)
+            assert (1 << lw) * (n2 + 1) <= R.LDS_TILE * (2 if np2 <= 2 else 1)
+            r = t0 + np.arange(cnt)
+            idx = r[:, None] * n2 + np.arange(n2)[None, :]
+            assert idx.max() < src_len
+            f = np.fft.fft(read_scr(idx), axis=1)
+            kk = r[:, None] + n1 * np.arange(n2)[None, :]
+            if kind == R.K_ROW:
+                assert kk.max() < dst_len
+                dst[do + kk] = (np.conj(f) if cj else f) * float(t["scale"])
+            elif kind == R.K_BLUE_ROW:
+                assert kk.max() < dst_len
+                B = _kernel_spectrum(n, m)
+                b = np.conj(B[(m - kk) & (m - 1)]) if inverse else B[kk]
+                dst[do + kk] = np.conj(f * b)
+                note_write(kk)
+            else:
+                assert kind == R.K_BLUE_OUT
+                ok = kk < n
+                w = _chirp(n, inverse)
+                dst[do + kk[ok]] = np.conj(f[ok]) * w[kk[ok]] * float(t["scale"])
+        if t["done"] >= 0:
+            done[int(t["done"])] = done.get(int(t["done"]), 0) + 1
+    return bufs[R.SRC_OUT]
+
+
+SIZES = [64, 97, 100, 4096, 4099, 5003, 8192, 12288, 59049, 65536, 65521, 60060, 131072, 3 * 5 * 7 * 11 * 13 * 8]
+
+
+@pytest.mark.parametrize("sync", ["counter", "launch"])
+@pytest.mark.parametrize("inverse", [False, True])
+@pytest.mark.parametrize("max_radix", [8, 16])
+def test_item_queue_computes_the_dft(inverse, max_radix, sync):
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal(sum(SIZES)) + 1j * rng.standard_normal(sum(SIZES))
+    q = R.plan_items(SIZES, inverse, True, max_radix, True, _FakeTables(), sync, tile_min=4097)
+    if sync == "launch":
+        assert len(q.segments) == 4 and (q.items["dep"] < 0).all() and (q.items["done"] < 0).all()
+    kinds = {(int(v) >> 16) & 0xF for v in q.items["npass"]}
+    assert kinds == {R.K_WHOLE, R.K_COL, R.K_ROW, R.K_BLUE_COL, R.K_BLUE_ROW, R.K_BLUE_OUT}
+    out = emulate(q, SIZES, x, inverse)
+    off = 0
+    for n in SIZES:
+        seg = x[off:off + n]
+        ref = np.fft.ifft(seg) if inverse else np.fft.fft(seg)
+        err = np.linalg.norm(out[off:off + n] - ref) / np.linalg.norm(ref)
+        assert err < 2e-7, f"n={n}: rel-L2 {err:.2e}"  # the records hold 1/n as float32
+        off += n
+
+
+def test_tile_min_keeps_shorter_direct_lengths_whole():
+    q = R.plan_items(SIZES, False, True, 16, True, _FakeTables(), "launch", tile_min=60000)
+    whole_n = {int(t["n"]) for t in q.items if ((int(t["npass"]) >> 16) & 0xF) == R.K_WHOLE}
+    tiled_direct = {int(t["n"]) for t in q.items if ((int(t["npass"]) >> 16) & 0xF) in (R.K_COL, R.K_ROW)
+                    and int(t["m"]) == 0}
+    assert tiled_direct == {n for n in SIZES if n >= 60000 and R.is_direct(n) and R.four_step_split(n)}
+    assert {8192, 12288, 59049} <= whole_n
+
+
+def test_item_queue_without_four_step_is_whole_transforms():
+    q = R.plan_items(SIZES, False, True, 16, False, _FakeTables())
+    assert all(((int(v) >> 16) & 0xF) == R.K_WHOLE for v in q.items["npass"])
+    assert len(q.items) == len(SIZES) and q.ncounters == 2
+
+
+def test_queue_order_check_rejects_a_wait_on_later_work():
+    q = R.plan_items([8192], False, True, 16, True, _FakeTables(), "counter", tile_min=4097)
+    items = q.items.copy()
+    R.check_queue_order(items)
+    with pytest.raises(AssertionError):
+        R.check_queue_order(items[::-1])
+
+
+def test_large_batch_tiles_fit_and_counters_are_dense():
+    sizes = R.mixed_sizes(512, seed=1)
+    q = R.plan_items(sizes, False, True, 16, True, _FakeTables(), "counter", tile_min=4097)
+    deps = q.items["dep"][q.items["dep"] >= 0]
+    dones = q.items["done"][q.items["done"] >= 0]
+    assert set(deps.tolist()) <= set(dones.tolist())
+    assert q.ncounters == 2 + len(set(dones.tolist()))
+    assert (q.items["count"] >= 1).all()

@@ -1,12 +1,29 @@
 """Ragged batches: many 1-D C2C transforms of different lengths in one launch.
 
 The transforms are packed back to back in one complex64 buffer (transform i
-occupies ``offsets[i] : offsets[i] + sizes[i]``).  ``RaggedFFT`` builds one
-task per transform -- a Stockham radix list for the length (radices 2..13 and
-16), or, for lengths with a prime factor above 13, a Bluestein plan (FP64 chirp,
-power-of-two chirp spectrum) -- and runs them all in one persistent launch of
-the ragged kernel (csrc/ragged.hip), longest first.  ``tune()`` picks the
-workgroup size and the largest radix by timing the actual batch.
+occupies ``offsets[i] : offsets[i] + sizes[i]``).  ``RaggedFFT`` turns the batch
+into a queue of work items for one persistent launch of the ragged kernel
+(csrc/ragged.hip):
+
+* lengths up to one LDS buffer (4096 points): one item per transform, a
+  Stockham radix list (radices 2..13 and 16) run in LDS;
+* longer lengths that split as L = n1 * n2 with both factors <= 512: the
+  four-step algorithm as column-tile items (length-n1 transforms of a run of
+  columns, times W_L^(k1 j2)) and row-tile items (length-n2 transforms of a run
+  of rows), so one long transform spreads over every workgroup instead of
+  running on one.  The levels run as consecutive launches of the same
+  persistent kernel (default), or in one launch where a row tile waits on its
+  transform's done counter;
+* lengths with a prime factor above 13: Bluestein (FP64 chirp, power-of-two
+  chirp spectrum), as four tile levels when the power of two splits;
+* anything else (longer than 512^2 without a split): one item with radix
+  passes through a per-workgroup scratch slice.
+
+With one launch the queue holds all first-level tiles, then each later level,
+then the whole transforms longest first; an item only waits for items queued
+before it.  With a launch per level the whole transforms join the first level.
+``tune()`` picks the workgroup size, the largest radix and the direct length
+from which transforms are tiled by timing the actual batch.
 
 This path also serves ``vd.fft.fft`` for lengths the single-workgroup kernels do
 not cover (prime factors above 31, or more than 16384 points).
@@ -15,7 +32,7 @@ from __future__ import annotations
 
 import ctypes
 import math
-from typing import Dict, List, Optional, Sequence, Tuple
+from typing import Dict, List, NamedTuple, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -26,10 +43,19 @@ RTASK_DTYPE = np.dtype([
     ("in", "<u8"), ("out", "<u8"), ("tw", "<u8"), ("chirp", "<u8"), ("bspec", "<u8"),
     ("n", "<i4"), ("m", "<i4"), ("npass", "<i4"), ("inverse", "<i4"),
     ("radix", "<i4", (12,)), ("scale", "<f4"), ("count", "<i4"),
+    ("tile0", "<i4"), ("dep", "<i4"), ("target", "<i4"), ("done", "<i4"),
 ])
-assert RTASK_DTYPE.itemsize == 112
+assert RTASK_DTYPE.itemsize == 128
+
+# item kinds (npass bits 16-19), the conjugate flag (bit 20) and log2 of the tile width
+# (bits 21-24), as in csrc/ragged.hip
+K_WHOLE, K_COL, K_ROW, K_BLUE_COL, K_BLUE_ROW, K_BLUE_OUT = range(6)
+CONJ = 1 << 20
 
 LDS_PINGPONG = 4096
+LDS_TILE = 4096 + 128  # complex64 elements of one LDS buffer (a tile at pitch n + 1)
+FOUR_STEP_MAX = 512  # longest four-step factor: tiles of >= 8 padded columns fit one LDS buffer
+TILE_MIN_DIRECT = 1 << 17  # direct lengths from here on are tiled (shorter ones run whole on one workgroup)
 DIRECT_PRIMES = (2, 3, 5, 7, 11, 13)
 
 
@@ -51,6 +77,53 @@ def runtime_radices(n: int, max_radix: int = 16) -> List[int]:
     if len(rads) > 12:
         raise ValueError(f"too many passes for n={n}")
     return rads
+
+
+def four_step_split(L: int, max_radix: int = 16) -> Optional[Tuple[List[int], List[int]]]:
+    """Radix lists of L = n1 * n2 (n1: column length, n2: row length, both <= 512) for
+    the four-step path of lengths above one LDS buffer; None when L fits in LDS or has
+    no such split.  Prime factors go to the smaller side, largest first."""
+    if L <= LDS_PINGPONG:
+        return None
+    if max(prime_factors(L)) > 13:
+        return None
+    a = b = 1
+    for p in sorted(prime_factors(L), reverse=True):
+        if a <= b:
+            a *= p
+        else:
+            b *= p
+    if max(a, b) > FOUR_STEP_MAX:
+        return None
+    r1, r2 = runtime_radices(a, max_radix), runtime_radices(b, max_radix)
+    if len(r1) + len(r2) > 12:
+        return None
+    return r1, r2
+
+
+def _encode_passes(t, radices: Sequence[int], split: Optional[Tuple[List[int], List[int]]], kind: int = K_WHOLE,
+                   flags: int = 0) -> None:
+    """Item pass fields: radices and npass, or for a four-step split the n1 list then the
+    n2 list with npass = total | (n1 passes << 8) | (kind << 16) | flags."""
+    if split is None:
+        t["npass"] = len(radices) | (kind << 16) | flags
+        t["radix"][:len(radices)] = radices
+    else:
+        r1, r2 = split
+        rads = list(r1) + list(r2)
+        t["npass"] = len(rads) | (len(r1) << 8) | (kind << 16) | flags
+        t["radix"][:len(rads)] = rads
+
+
+def tile_log2_width(n: int, npass: int = 3) -> int:
+    """log2 of the tile width for length-n transforms: the largest power of two W with
+    W (n + 1) <= one LDS buffer, or two buffers when the tile has at most two passes
+    (its first pass reads global memory and its last writes it, so one LDS copy is
+    live at a time; the LDS pitch is n + 1)."""
+    w = (LDS_TILE * (2 if npass <= 2 else 1)) // (n + 1)
+    if w < 1:
+        raise ValueError(f"length {n} does not fit an LDS tile")
+    return w.bit_length() - 1
 
 
 def is_direct(n: int) -> bool:
@@ -85,6 +158,19 @@ class _Tables:
                 self.tw[key] = p
         return self.tw[key]
 
+    def twiddles_split(self, split: Tuple[List[int], List[int]]) -> int:
+        """n1's Stockham table followed by n2's (one allocation)."""
+        r1, r2 = (tuple(r) for r in split)
+        key = ("split", r1, r2)
+        if key not in self.tw:
+            n1, n2 = int(np.prod(r1)), int(np.prod(r2))
+            p = rt.malloc((n1 - 1 + n2 - 1) * 8)
+            for off, n, rads in ((0, n1, r1), ((n1 - 1) * 8, n2, r2)):
+                arr = (ctypes.c_int * len(rads))(*rads)
+                rt.check(rt.lib().vkx_twiddles(p + off, n, arr, len(rads), 0, rt.current_stream()), "vkx_twiddles")
+            self.tw[key] = p
+        return self.tw[key]
+
     def chirp_table(self, n: int) -> int:
         if n not in self.chirp:
             p = rt.malloc(n * 8)
@@ -103,7 +189,7 @@ class _Tables:
             t["tw"] = self.twiddles(m, radices)
             t["n"], t["m"], t["npass"], t["inverse"] = m, 0, len(radices), 0
             t["radix"][0, :len(radices)] = radices
-            t["scale"], t["count"] = 1.0, 1
+            t["scale"], t["count"], t["dep"], t["done"] = 1.0, 1, -1, -1
             _launch_tasks(t, max(1, 2 * m) if m > LDS_PINGPONG else 1, grid=1, block=256)
             self.bspec[key] = p
         return self.bspec[key]
@@ -145,7 +231,7 @@ _arena = _Arena()
 
 
 def _launch_tasks(tasks: np.ndarray, scratch_elems_per_wg: int, grid: int, block: int, stream=None,
-                  task_buf=None) -> None:
+                  task_buf=None, counters: Optional[Tuple[int, int]] = None, spin_limit: int = 0) -> None:
     s = rt.current_stream() if stream is None else int(stream)
     if task_buf is None:
         task_ptr = rt.malloc(tasks.nbytes)
@@ -153,8 +239,9 @@ def _launch_tasks(tasks: np.ndarray, scratch_elems_per_wg: int, grid: int, block
     else:
         task_ptr = task_buf
     scr = _arena.ensure(max(1, scratch_elems_per_wg) * 8 * grid)
-    rc = rt.lib().vkx_ragged_exec(task_ptr, len(tasks), _arena.counter_ptr(), scr, scratch_elems_per_wg, grid, block,
-                                  s)
+    ctr, nctr = counters if counters is not None else (_arena.counter_ptr(), 2)
+    rc = rt.lib().vkx_ragged_exec(task_ptr, len(tasks), ctr, nctr, scr, scratch_elems_per_wg, grid, block,
+                                  int(spin_limit), s)
     if rc != 0:
         raise RuntimeError("vkx_ragged_exec launch failed")
     if task_buf is None:
@@ -162,16 +249,154 @@ def _launch_tasks(tasks: np.ndarray, scratch_elems_per_wg: int, grid: int, block
         rt.free(task_ptr)
 
 
+def check_queue_order(items: np.ndarray) -> None:
+    """Every item that waits on a counter comes after all the items that raise that
+    counter to its target (so a wait is only ever for work fetched earlier)."""
+    seen: Dict[int, int] = {}
+    for k, t in enumerate(items):
+        dep = int(t["dep"])
+        if dep >= 0 and seen.get(dep, 0) != int(t["target"]):
+            raise AssertionError(f"item {k} waits for counter {dep} = {int(t['target'])}, "
+                                 f"{seen.get(dep, 0)} producers queued before it")
+        done = int(t["done"])
+        if done >= 0:
+            seen[done] = seen.get(done, 0) + 1
+
+
 def _task_cost(n: int, m: int) -> float:
     L = m if m else n
     return L * max(1.0, math.log2(max(L, 2))) * (3.0 if m else 1.0)
 
 
+SRC_IN, SRC_OUT, SRC_SCR = 0, 1, 2  # item pointer bases: the input, the output, the plan's scratch
+
+
+class ItemQueue(NamedTuple):
+    items: np.ndarray      # RTASK_DTYPE records in queue order (in / out filled at bind time)
+    src: np.ndarray        # (base, element offset) of each item's input
+    dst: np.ndarray        # (base, element offset) of each item's output
+    ncounters: int         # queue position + abort flag + done counters
+    scratch_per_wg: int    # complex64 scratch per workgroup for whole-transform items
+    scratch_elems: int     # complex64 scratch of the plan's tiled transforms
+    segments: List[Tuple[int, int]]  # item ranges launched one after another
+
+
+def plan_items(sizes: Sequence[int], inverse: bool, normalize: bool, mr: int, four_step: bool, T,
+               sync: str = "launch", tile_min: int = TILE_MIN_DIRECT) -> ItemQueue:
+    """Work-item queue of a ragged batch.  ``T`` provides the device tables
+    (twiddles, twiddles_split, chirp_table, chirp_spectrum) as addresses.
+    sync="launch" (default): one launch per tile level (the kernel boundary orders the
+    levels, no waits), whole transforms in the first;
+    sync="counter": one launch, later tile levels wait on done counters (each item's
+    agent-scope release / acquire writes back and invalidates L2, which costs more
+    than the launches it saves on this 8-XCD device).  Direct lengths >= tile_min and
+    Bluestein lengths are tiled when four_step is set and the length splits."""
+    if sync not in ("counter", "launch"):
+        raise ValueError("sync is 'counter' or 'launch'")
+    sizes = np.asarray(sizes, dtype=np.int64)
+    offsets = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+
+    def counter() -> int:
+        nctr[0] += 1
+        return nctr[0] - 3
+
+    levels: List[list] = [[], [], [], []]  # (record, (src, off), (dst, off)) per tile level
+    whole: list = []                       # (cost, record, offset)
+    nctr = [2]
+    scr_elems = 0
+    scratch = 1
+    cost = [_task_cost(n, 0 if is_direct(n) else bluestein_length(n)) for n in sizes.tolist()]
+    I, O, S = SRC_IN, SRC_OUT, SRC_SCR
+    for i in sorted(range(len(sizes)), key=lambda j: -cost[j]):
+        n, off = int(sizes[i]), int(offsets[i])
+        base = np.zeros((), RTASK_DTYPE)
+        base["n"], base["inverse"], base["count"], base["dep"], base["done"] = n, int(inverse), 1, -1, -1
+        base["scale"] = (1.0 / n) if (inverse and normalize) else 1.0
+        direct = is_direct(n)
+        L = n if direct else bluestein_length(n)
+        rads = runtime_radices(L, mr)  # [] for n == 1: in place, nothing to do
+        split = four_step_split(L, mr) if four_step and (not direct or n >= tile_min) else None
+        if not direct:
+            base["m"] = L
+            base["chirp"] = T.chirp_table(n)
+            base["bspec"] = T.chirp_spectrum(n, L, rads)
+        if split is None:
+            rec = base.copy()
+            _encode_passes(rec, rads, None)
+            rec["tw"] = T.twiddles(L, rads) if rads else 0
+            if not direct:
+                scratch = max(scratch, 3 * L)
+            elif n > LDS_PINGPONG:
+                scratch = max(scratch, 2 * n)
+            whole.append((cost[i], rec, off))
+            continue
+        r1, r2 = split
+        n1 = int(np.prod(r1))
+        n2 = L // n1
+        lc, lr = tile_log2_width(n1, len(r1)), tile_log2_width(n2, len(r2))
+        cw, rw = 1 << lc, 1 << lr
+        ncol, nrow = -(-n2 // cw), -(-n1 // rw)
+        base["tw"] = T.twiddles_split(split)
+
+        def tiles(kind, flags, src, dst, dep, target, done, scale=None, cols=True):
+            span, width = (n2, cw) if cols else (n1, rw)
+            for t0 in range(0, span, width):
+                rec = base.copy()
+                _encode_passes(rec, rads, split, kind, flags | ((lc if cols else lr) << 21))
+                rec["tile0"], rec["count"] = t0, min(width, span - t0)
+                rec["dep"], rec["target"], rec["done"] = dep, target, done
+                if scale is not None:
+                    rec["scale"] = scale
+                yield rec, src, dst
+
+        conj = CONJ if inverse else 0
+        if direct:
+            c1 = counter()
+            y = scr_elems
+            scr_elems += L
+            levels[0] += tiles(K_COL, conj, (I, off), (S, y), -1, 0, c1)
+            levels[1] += tiles(K_ROW, conj, (S, y), (O, off), c1, ncol, -1, cols=False)
+        else:
+            c1, c2, c3 = counter(), counter(), counter()
+            a, b = scr_elems, scr_elems + L
+            scr_elems += 2 * L
+            levels[0] += tiles(K_BLUE_COL, 0, (I, off), (S, a), -1, 0, c1)
+            levels[1] += tiles(K_BLUE_ROW, 0, (S, a), (S, b), c1, ncol, c2, cols=False)
+            levels[2] += tiles(K_COL, 0, (S, b), (S, a), c2, nrow, c3)
+            levels[3] += tiles(K_BLUE_OUT, 0, (S, a), (O, off), c3, ncol, -1,
+                               scale=float(base["scale"]) / L, cols=False)
+    wq = [(rec, (I, off), (O, off)) for _, rec, off in sorted(whole, key=lambda w: -w[0])]
+    if sync == "counter":
+        queue = [x for lv in levels for x in lv] + wq
+        segments = [(0, len(queue))]
+    else:
+        for lv in levels:
+            for rec, _, _ in lv:
+                rec["dep"], rec["target"], rec["done"] = -1, 0, -1
+        nctr[0] = 2
+        groups = [levels[0] + wq] + [lv for lv in levels[1:] if lv]
+        queue = [x for g in groups for x in g]
+        bounds = np.cumsum([0] + [len(g) for g in groups]).tolist()
+        segments = [(a, b) for a, b in zip(bounds[:-1], bounds[1:]) if b > a]
+    items = np.zeros(len(queue), RTASK_DTYPE)
+    src_ref = np.zeros((len(queue), 2), np.int64)
+    dst_ref = np.zeros((len(queue), 2), np.int64)
+    for k, (rec, src, dst) in enumerate(queue):
+        items[k] = rec
+        src_ref[k], dst_ref[k] = src, dst
+    check_queue_order(items)
+    return ItemQueue(items, src_ref, dst_ref, nctr[0], scratch, scr_elems, segments)
+
+
 class RaggedFFT:
-    """Plan for a packed batch of transforms of lengths ``sizes`` (forward or inverse)."""
+    """Plan for a packed batch of transforms of lengths ``sizes`` (forward or inverse).
+    ``four_step`` (default): lengths above one LDS buffer that split into two factors
+    <= 512 run as column-tile and row-tile items spread over all workgroups; without
+    it they run whole on one workgroup through a scratch slice."""
 
     def __init__(self, sizes: Sequence[int], inverse: bool = False, normalize: bool = True, block: int = 256,
-                 max_radix: int = 16, grid: Optional[int] = None):
+                 max_radix: int = 16, grid: Optional[int] = None, four_step: bool = True, spin_limit: int = 0,
+                 sync: str = "launch", tile_min: int = TILE_MIN_DIRECT):
         self.sizes = np.asarray(sizes, dtype=np.int64)
         if np.any(self.sizes < 1):
             raise ValueError("transform lengths must be positive")
@@ -181,51 +406,40 @@ class RaggedFFT:
         self.normalize = bool(normalize)
         self.block = int(block)
         self.max_radix = int(max_radix)
+        self.spin_limit = int(spin_limit)
         dev = rt.ensure_device()
-        self.grid = max(1, min(int(grid) if grid else 2 * int(dev.compute_units), len(self.sizes)))
-        T = tables()
-        self._template = np.zeros(len(self.sizes), RTASK_DTYPE)
-        scratch = 1
-        for i, n in enumerate(self.sizes.tolist()):
-            t = self._template[i]
-            t["n"], t["inverse"], t["count"] = n, int(self.inverse), 1
-            t["scale"] = (1.0 / n) if (self.inverse and self.normalize) else 1.0
-            if is_direct(n):
-                rads = runtime_radices(n, self.max_radix)  # [] for n == 1: in place, nothing to do
-                t["m"], t["npass"] = 0, len(rads)
-                t["radix"][:len(rads)] = rads
-                t["tw"] = T.twiddles(n, rads) if rads else 0
-                if n > LDS_PINGPONG:
-                    scratch = max(scratch, 2 * n)
-            else:
-                m = bluestein_length(n)
-                rads = runtime_radices(m, self.max_radix)
-                t["m"], t["npass"] = m, len(rads)
-                t["radix"][:len(rads)] = rads
-                t["tw"] = T.twiddles(m, rads)
-                t["chirp"] = T.chirp_table(n)
-                t["bspec"] = T.chirp_spectrum(n, m, rads)
-                scratch = max(scratch, 3 * m)
+        q = plan_items(self.sizes, self.inverse, self.normalize, self.max_radix, four_step, tables(), sync, tile_min)
+        self._items, self._src, self._dst = q.items, q.src, q.dst
+        self._segments = q.segments
+        self._nctr = q.ncounters
+        scratch, scr_elems = q.scratch_per_wg, q.scratch_elems
         self.scratch_per_wg = scratch
-        cost = np.array([_task_cost(int(t["n"]), int(t["m"])) for t in self._template])
-        self.order = np.argsort(-cost, kind="stable")
+        self.scratch_elems = scr_elems
+        self.grid = max(1, min(int(grid) if grid else 2 * int(dev.compute_units), len(self._items)))
+        self._scr = rt.malloc(max(1, scr_elems) * 8)
+        self._ctr = rt.malloc(4 * self._nctr)
         self._task_cache: Dict[Tuple[int, int], int] = {}
+
+    @property
+    def n_items(self) -> int:
+        return len(self._items)
 
     def _tasks_for(self, in_ptr: int, out_ptr: int) -> int:
         key = (in_ptr, out_ptr)
         if key not in self._task_cache:
-            t = self._template[self.order].copy()
-            off = self.offsets[self.order] * 8
-            t["in"] = in_ptr + off
-            t["out"] = out_ptr + off
+            t = self._items.copy()
+            bases = np.array([in_ptr, out_ptr, self._scr], dtype=np.uint64)
+            t["in"] = bases[self._src[:, 0]] + (self._src[:, 1] * 8).astype(np.uint64)
+            t["out"] = bases[self._dst[:, 0]] + (self._dst[:, 1] * 8).astype(np.uint64)
             p = rt.malloc(t.nbytes)
             rt.htod(p, t.ctypes.data, t.nbytes)
             self._task_cache[key] = p
         return self._task_cache[key]
 
     def bind(self, out_buf, in_buf=None):
-        """Upload the task table for this (input, output) pair and size the scratch
-        arena; returns ``launch(stream)`` that allocates nothing (safe under graph capture)."""
+        """Upload the item table for this (input, output) pair and size the scratch
+        arena; returns ``launch(stream)`` that allocates nothing (safe under graph capture).
+        Launches of one plan share its counters and scratch: they must not overlap."""
         if in_buf is None:
             in_buf = out_buf
         for b in (out_buf, in_buf):
@@ -235,18 +449,40 @@ class RaggedFFT:
             raise ValueError("out-of-place ragged batches may not contain length-1 transforms")
         tp = self._tasks_for(int(in_buf.ptr), int(out_buf.ptr))
         _arena.ensure(max(1, self.scratch_per_wg) * 8 * self.grid)
-        _arena.counter_ptr()
         keep = (out_buf, in_buf)
 
         def launch(stream=None):
             _ = keep
-            _launch_tasks(self._template, self.scratch_per_wg, self.grid, self.block, stream, task_buf=tp)
+            for a, b in self._segments:
+                _launch_tasks(self._items[a:b], self.scratch_per_wg, self.grid, self.block, stream,
+                              task_buf=tp + a * RTASK_DTYPE.itemsize, counters=(self._ctr, self._nctr),
+                              spin_limit=self.spin_limit)
         return launch
 
     def __call__(self, out_buf, in_buf=None, stream=None) -> None:
         """Transform ``in_buf`` (default: ``out_buf``, in place) into ``out_buf``;
         both complex64 buffers of ``total`` elements."""
         self.bind(out_buf, in_buf)(stream)
+
+    def __del__(self):
+        try:
+            ptrs = [getattr(self, "_scr", 0), getattr(self, "_ctr", 0)] + list(getattr(self, "_task_cache", {}).values())
+            if any(ptrs):
+                rt.synchronize()
+            for p in ptrs:
+                if p:
+                    rt.free(p)
+        except Exception:
+            pass
+
+    def aborted(self) -> bool:
+        """True when the last launch gave up a dependency wait (its result is invalid).
+        Synchronises the device."""
+        rt.synchronize()
+        flag = np.zeros(1, np.int32)
+        rt.dtoh(flag.ctypes.data, self._ctr + 4, 4)
+        rt.synchronize()
+        return bool(flag[0])
 
     def flops(self) -> float:
         return float(np.sum(5.0 * self.sizes * np.log2(np.maximum(self.sizes, 2))))
@@ -307,22 +543,31 @@ def packed_normal(sizes: Sequence[int], seed: int = 0):
     return parts, np.concatenate(parts).astype(np.complex64)
 
 
+TUNE_TILE_MIN = (4097, 16385, TILE_MIN_DIRECT)  # tile direct lengths above one LDS buffer / > 16K / never
+
+
 def tune(sizes: Sequence[int], buf, reps: int = 5) -> Dict:
-    """Pick the workgroup size and largest radix for this batch by timing."""
+    """Pick the workgroup size, the largest radix and the direct length from which
+    transforms are tiled for this batch by timing each candidate plan (deterministic
+    candidate order; ties keep the earlier candidate).  The best entry's keys are
+    RaggedFFT arguments."""
     best = None
     results = []
     for block in (256, 512):
         for max_radix in (8, 16):
-            plan = RaggedFFT(sizes, block=block, max_radix=max_radix)
-            plan(buf)
-            e0, e1 = rt.Event(), rt.Event()
-            e0.record()
-            for _ in range(reps):
-                plan(buf)
-            e1.record()
-            e1.synchronize()
-            ms = e0.elapsed_ms(e1) / reps
-            results.append({"block": block, "max_radix": max_radix, "ms": ms})
-            if best is None or ms < best["ms"]:
-                best = results[-1]
+            for tile_min in TUNE_TILE_MIN:
+                plan = RaggedFFT(sizes, block=block, max_radix=max_radix, tile_min=tile_min)
+                run = plan.bind(buf)
+                run()
+                e0, e1 = rt.Event(), rt.Event()
+                e0.record()
+                for _ in range(reps):
+                    run()
+                e1.record()
+                e1.synchronize()
+                ms = e0.elapsed_ms(e1) / reps
+                results.append({"block": block, "max_radix": max_radix, "tile_min": tile_min, "ms": ms})
+                if best is None or ms < best["ms"]:
+                    best = results[-1]
+                del run, plan
     return {"best": best, "results": results}

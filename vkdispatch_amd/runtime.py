@@ -98,7 +98,8 @@ _SIGS = {
     "vkx_batch_sum": ([_U64, _U64, ctypes.c_int, ctypes.c_longlong, ctypes.c_int, _U64], ctypes.c_int),
     "vkx_chirp": ([_U64, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _U64], ctypes.c_int),
     "vkx_xcc_map": ([ctypes.POINTER(ctypes.c_int), ctypes.c_int], ctypes.c_int),
-    "vkx_ragged_exec": ([_U64, ctypes.c_int, _U64, _U64, ctypes.c_longlong, ctypes.c_int, ctypes.c_int, _U64],
+    "vkx_ragged_exec": ([_U64, ctypes.c_int, _U64, ctypes.c_int, _U64, ctypes.c_longlong, ctypes.c_int, ctypes.c_int,
+                         ctypes.c_int, _U64],
                         ctypes.c_int),
     "vkx_bluestein_b": ([_U64, ctypes.c_int, ctypes.c_int, _U64], ctypes.c_int),
     "vkx_graph_begin": ([_U64], ctypes.c_int),
