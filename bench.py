@@ -232,9 +232,9 @@ def bench_ragged(steps, warmup, args):
     parts, packed = R.packed_normal(sizes, 0)
     src = vd.asbuffer(packed)
     buf = vd.Buffer((packed.size,), vd.complex64)
-    tuned = R.tune(sizes, buf, reps=3)
+    tuned = R.tune(sizes, buf, reps=3)  # timed live; tuned_plan() would reuse the stored pick
     best = tuned["best"]
-    plan = R.RaggedFFT(sizes, block=best["block"], max_radix=best["max_radix"], tile_min=best["tile_min"])
+    plan = R.RaggedFFT(sizes, **{k: best[k] for k in R.TUNE_KEYS})
     g = vd.CommandGraph()
     g.record_callable(plan.bind(buf, src))
     g.capture()

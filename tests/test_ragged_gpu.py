@@ -129,3 +129,14 @@ def test_ragged_wait_gives_up_and_the_launch_ends():
     buf.write(packed)
     ok(buf)
     assert not ok.aborted()
+
+
+def test_tune_is_cached_per_batch(tmp_path):
+    sizes = ragged_sizes(48, seed=5, hi=20000)
+    _, packed = _packed(sizes, 5)
+    buf = vd.asbuffer(packed)
+    path = str(tmp_path / "tuned.json")
+    first = R.tune(sizes, buf, reps=1, cache=True, save=True, path=path)
+    assert not first["cached"] and len(first["results"]) == 18
+    again = R.tune(sizes, buf, reps=1, cache=True, path=path)
+    assert again["cached"] and {k: again["best"][k] for k in R.TUNE_KEYS} == {k: first["best"][k] for k in R.TUNE_KEYS}

@@ -191,3 +191,15 @@ def test_large_batch_tiles_fit_and_counters_are_dense():
     assert set(deps.tolist()) <= set(dones.tolist())
     assert q.ncounters == 2 + len(set(dones.tolist()))
     assert (q.items["count"] >= 1).all()
+
+
+def test_tune_table_key_and_round_trip(tmp_path):
+    a = R.mixed_sizes(64, seed=3)
+    assert R.batch_key(a) == R.batch_key(list(a)) and R.batch_key(a) != R.batch_key(a[::-1])
+    path = str(tmp_path / "t.json")
+    assert R.load_tune_table(path) == {}
+    R.save_tune_table({"gfx950|" + R.batch_key(a): {"block": 256, "block_first": 512, "max_radix": 16,
+                                                   "tile_min": 4097}}, path)
+    assert R.load_tune_table(path)["gfx950|" + R.batch_key(a)]["tile_min"] == 4097
+    shipped = R.load_tune_table()
+    assert all(set(v) == set(R.TUNE_KEYS) for v in shipped.values())

@@ -23,7 +23,9 @@ With one launch the queue holds all first-level tiles, then each later level,
 then the whole transforms longest first; an item only waits for items queued
 before it.  With a launch per level the whole transforms join the first level.
 ``tune()`` picks the workgroup size, the largest radix and the direct length
-from which transforms are tiled by timing the actual batch.
+from which transforms are tiled by timing the actual batch; ``tuned_plan()``
+keeps the pick in ``tuned_ragged.json`` (keyed by device arch and a hash of the
+length list) so a batch is timed once per device.
 
 This path also serves ``vd.fft.fft`` for lengths the single-workgroup kernels do
 not cover (prime factors above 31, or more than 16384 points).
@@ -31,7 +33,10 @@ not cover (prime factors above 31, or more than 16384 points).
 from __future__ import annotations
 
 import ctypes
+import hashlib
+import json
 import math
+import os
 from typing import Dict, List, NamedTuple, Optional, Sequence, Tuple
 
 import numpy as np
@@ -396,7 +401,7 @@ class RaggedFFT:
 
     def __init__(self, sizes: Sequence[int], inverse: bool = False, normalize: bool = True, block: int = 256,
                  max_radix: int = 16, grid: Optional[int] = None, four_step: bool = True, spin_limit: int = 0,
-                 sync: str = "launch", tile_min: int = TILE_MIN_DIRECT):
+                 sync: str = "launch", tile_min: int = TILE_MIN_DIRECT, block_first: Optional[int] = None):
         self.sizes = np.asarray(sizes, dtype=np.int64)
         if np.any(self.sizes < 1):
             raise ValueError("transform lengths must be positive")
@@ -405,6 +410,7 @@ class RaggedFFT:
         self.inverse = bool(inverse)
         self.normalize = bool(normalize)
         self.block = int(block)
+        self.block_first = int(block_first) if block_first else self.block  # first launch (holds whole items)
         self.max_radix = int(max_radix)
         self.spin_limit = int(spin_limit)
         dev = rt.ensure_device()
@@ -453,8 +459,9 @@ class RaggedFFT:
 
         def launch(stream=None):
             _ = keep
-            for a, b in self._segments:
-                _launch_tasks(self._items[a:b], self.scratch_per_wg, self.grid, self.block, stream,
+            for i, (a, b) in enumerate(self._segments):
+                _launch_tasks(self._items[a:b], self.scratch_per_wg, self.grid, self.block_first if i == 0 else self.block,
+                              stream,
                               task_buf=tp + a * RTASK_DTYPE.itemsize, counters=(self._ctr, self._nctr),
                               spin_limit=self.spin_limit)
         return launch
@@ -544,19 +551,47 @@ def packed_normal(sizes: Sequence[int], seed: int = 0):
 
 
 TUNE_TILE_MIN = (4097, 16385, TILE_MIN_DIRECT)  # tile direct lengths above one LDS buffer / > 16K / never
+TUNE_KEYS = ("block", "block_first", "max_radix", "tile_min")  # RaggedFFT arguments chosen by tune()
+TUNE_TABLE_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_ragged.json")
 
 
-def tune(sizes: Sequence[int], buf, reps: int = 5) -> Dict:
-    """Pick the workgroup size, the largest radix and the direct length from which
+def batch_key(sizes: Sequence[int]) -> str:
+    """Cache key of a ragged batch: its length and a hash of its length list (in order)."""
+    a = np.ascontiguousarray(np.asarray(sizes, dtype=np.int64))
+    return f"{a.size}:{hashlib.sha1(a.tobytes()).hexdigest()[:16]}"
+
+
+def load_tune_table(path: str = TUNE_TABLE_PATH) -> Dict:
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return {}
+
+
+def save_tune_table(table: Dict, path: str = TUNE_TABLE_PATH) -> None:
+    with open(path, "w") as f:
+        json.dump(table, f, indent=1, sort_keys=True)
+
+
+def tune(sizes: Sequence[int], buf, reps: int = 5, cache: bool = False, save: bool = False,
+         path: str = TUNE_TABLE_PATH) -> Dict:
+    """Pick the workgroup sizes (first launch, which holds the whole transforms, and
+    the tile launches), the largest radix and the direct length from which
     transforms are tiled for this batch by timing each candidate plan (deterministic
     candidate order; ties keep the earlier candidate).  The best entry's keys are
-    RaggedFFT arguments."""
+    RaggedFFT arguments.  cache: return the entry stored for (device arch, batch) in
+    the table at ``path`` without timing, when there is one; save: store the result."""
+    key = f"{rt.device_arch()}|{batch_key(sizes)}"
+    if cache:
+        hit = load_tune_table(path).get(key)
+        if hit is not None:
+            return {"best": dict(hit), "results": [], "cached": True}
     best = None
     results = []
-    for block in (256, 512):
+    for block, block_first in ((256, 256), (256, 512), (512, 512)):
         for max_radix in (8, 16):
             for tile_min in TUNE_TILE_MIN:
-                plan = RaggedFFT(sizes, block=block, max_radix=max_radix, tile_min=tile_min)
+                plan = RaggedFFT(sizes, block=block, max_radix=max_radix, tile_min=tile_min, block_first=block_first)
                 run = plan.bind(buf)
                 run()
                 e0, e1 = rt.Event(), rt.Event()
@@ -566,8 +601,19 @@ def tune(sizes: Sequence[int], buf, reps: int = 5) -> Dict:
                 e1.record()
                 e1.synchronize()
                 ms = e0.elapsed_ms(e1) / reps
-                results.append({"block": block, "max_radix": max_radix, "tile_min": tile_min, "ms": ms})
+                results.append({"block": block, "block_first": block_first, "max_radix": max_radix,
+                                "tile_min": tile_min, "ms": ms})
                 if best is None or ms < best["ms"]:
                     best = results[-1]
                 del run, plan
-    return {"best": best, "results": results}
+    if save:
+        table = load_tune_table(path)
+        table[key] = {k: best[k] for k in TUNE_KEYS}
+        save_tune_table(table, path)
+    return {"best": best, "results": results, "cached": False}
+
+
+def tuned_plan(sizes: Sequence[int], buf, inverse: bool = False, normalize: bool = True) -> "RaggedFFT":
+    """RaggedFFT with the cached (or freshly measured and stored) tuning for this batch."""
+    best = tune(sizes, buf, cache=True, save=True)["best"]
+    return RaggedFFT(sizes, inverse=inverse, normalize=normalize, **{k: best[k] for k in TUNE_KEYS})
