@@ -232,7 +232,7 @@ def bench_ragged(steps, warmup, args):
     parts, packed = R.packed_normal(sizes, 0)
     src = vd.asbuffer(packed)
     buf = vd.Buffer((packed.size,), vd.complex64)
-    tuned = R.tune(sizes, buf, reps=3)  # timed live; tuned_plan() would reuse the stored pick
+    tuned = R.tune(sizes, buf, reps=3, cache=bool(getattr(args, "cached_tune", False)))
     best = tuned["best"]
     plan = R.RaggedFFT(sizes, **{k: best[k] for k in R.TUNE_KEYS})
     g = vd.CommandGraph()
@@ -417,6 +417,8 @@ def main():
     ap.add_argument("--workload", default="conv2d", choices=["conv2d", "c2c4096", "ragged", "mixed3d", "autograd", "copy"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-mixed", action="store_true", help="conv2d: skip the BF16-MFMA 256^3 comparison")
+    ap.add_argument("--cached-tune", action="store_true",
+                    help="ragged: use the stored autotuner pick for the batch instead of timing the candidates")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "vkdispatch_amd benchmarks one GPU per process; multi-GPU sharding is "
