@@ -203,3 +203,10 @@ def test_tune_table_key_and_round_trip(tmp_path):
     assert R.load_tune_table(path)["gfx950|" + R.batch_key(a)]["tile_min"] == 4097
     shipped = R.load_tune_table()
     assert all(set(v) == set(R.TUNE_KEYS) for v in shipped.values())
+
+
+def test_auto_tile_min():
+    assert R.auto_tile_min([65536] * 2, 512) == 16384
+    assert R.auto_tile_min([8192] * 32, 512) > 8192  # short lengths stay whole
+    assert R.auto_tile_min(R.mixed_sizes(4096, seed=0), 512) == R.TILE_MIN_DIRECT
+    assert R.auto_tile_min([65536] * 600, 512) == R.TILE_MIN_DIRECT  # enough long transforms to fill the GPU

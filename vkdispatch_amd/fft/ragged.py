@@ -120,6 +120,16 @@ def _encode_passes(t, radices: Sequence[int], split: Optional[Tuple[List[int], L
         t["radix"][:len(rads)] = rads
 
 
+def auto_tile_min(sizes: Sequence[int], resident_wgs: int) -> int:
+    """Default direct length from which transforms are tiled: 16384 when fewer such
+    transforms than resident workgroups exist (whole, most of the GPU would idle:
+    2 x 65536 points take 133 us whole and 40 us tiled), else never (a batch with
+    enough long transforms keeps every workgroup busy on whole transforms, which
+    then cost less than the tiles' two extra passes)."""
+    n_long = sum(1 for n in sizes if n >= 16384 and is_direct(int(n)))
+    return 16384 if n_long < resident_wgs else TILE_MIN_DIRECT
+
+
 def tile_log2_width(n: int, npass: int = 3) -> int:
     """log2 of the tile width for length-n transforms: the largest power of two W with
     W (n + 1) <= one LDS buffer, or two buffers when the tile has at most two passes
@@ -401,7 +411,7 @@ class RaggedFFT:
 
     def __init__(self, sizes: Sequence[int], inverse: bool = False, normalize: bool = True, block: int = 256,
                  max_radix: int = 16, grid: Optional[int] = None, four_step: bool = True, spin_limit: int = 0,
-                 sync: str = "launch", tile_min: int = TILE_MIN_DIRECT, block_first: Optional[int] = None):
+                 sync: str = "launch", tile_min: Optional[int] = None, block_first: Optional[int] = None):
         self.sizes = np.asarray(sizes, dtype=np.int64)
         if np.any(self.sizes < 1):
             raise ValueError("transform lengths must be positive")
@@ -414,7 +424,11 @@ class RaggedFFT:
         self.max_radix = int(max_radix)
         self.spin_limit = int(spin_limit)
         dev = rt.ensure_device()
-        q = plan_items(self.sizes, self.inverse, self.normalize, self.max_radix, four_step, tables(), sync, tile_min)
+        if tile_min is None:
+            tile_min = auto_tile_min(self.sizes, 2 * int(dev.compute_units))
+        self.tile_min = int(tile_min)
+        q = plan_items(self.sizes, self.inverse, self.normalize, self.max_radix, four_step, tables(), sync,
+                       self.tile_min)
         self._items, self._src, self._dst = q.items, q.src, q.dst
         self._segments = q.segments
         self._nctr = q.ncounters
