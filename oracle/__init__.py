@@ -1,7 +1,7 @@
 """FP64 CPU oracle of the vd.fft path -- TEST INFRASTRUCTURE ONLY.
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline
-leg may import this package, and only as the checker / baseline.  The product
+leg may import this package, and only to check results or as the CPU baseline.  The product
 (``vkdispatch_amd``) never imports it and has no CPU fallback.
 
 Contents (see ``fft_oracle.py`` for per-function reference citations):
