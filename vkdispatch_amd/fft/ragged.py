@@ -21,7 +21,7 @@ into a queue of work items for one persistent launch of the ragged kernel
 
 With one launch the queue holds all first-level tiles, then each later level,
 then the whole transforms longest first; an item only waits for items queued
-before it.  With a launch per level the whole transforms join the first level.
+before it.  With a launch per level the whole transforms head the first level's queue.
 ``tune()`` picks the workgroup size, the largest radix and the direct length
 from which transforms are tiled by timing the actual batch; ``tuned_plan()``
 keeps the pick in ``tuned_ragged.json`` (keyed by device arch and a hash of the
@@ -301,7 +301,7 @@ def plan_items(sizes: Sequence[int], inverse: bool, normalize: bool, mr: int, fo
     """Work-item queue of a ragged batch.  ``T`` provides the device tables
     (twiddles, twiddles_split, chirp_table, chirp_spectrum) as addresses.
     sync="launch" (default): one launch per tile level (the kernel boundary orders the
-    levels, no waits), whole transforms in the first;
+    levels, no waits), whole transforms at the head of the first;
     sync="counter": one launch, later tile levels wait on done counters (each item's
     agent-scope release / acquire writes back and invalidates L2, which costs more
     than the launches it saves on this 8-XCD device).  Direct lengths >= tile_min and
@@ -389,7 +389,9 @@ def plan_items(sizes: Sequence[int], inverse: bool, normalize: bool, mr: int, fo
             for rec, _, _ in lv:
                 rec["dep"], rec["target"], rec["done"] = -1, 0, -1
         nctr[0] = 2
-        groups = [levels[0] + wq] + [lv for lv in levels[1:] if lv]
+        # the whole transforms go first: the longest of them (one workgroup each) set the
+        # first launch's critical path, the short first-level tiles fill in behind them
+        groups = [wq + levels[0]] + [lv for lv in levels[1:] if lv]
         queue = [x for g in groups for x in g]
         bounds = np.cumsum([0] + [len(g) for g in groups]).tolist()
         segments = [(a, b) for a, b in zip(bounds[:-1], bounds[1:]) if b > a]
@@ -564,7 +566,7 @@ def packed_normal(sizes: Sequence[int], seed: int = 0):
     return parts, np.concatenate(parts).astype(np.complex64)
 
 
-TUNE_TILE_MIN = (4097, 16385, TILE_MIN_DIRECT)  # tile direct lengths above one LDS buffer / > 16K / never
+TUNE_TILE_MIN = (4097, 16385, 32769, 65536, TILE_MIN_DIRECT)  # direct lengths tiled from ... / never
 TUNE_KEYS = ("block", "block_first", "max_radix", "tile_min")  # RaggedFFT arguments chosen by tune()
 TUNE_TABLE_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_ragged.json")
 
