@@ -74,6 +74,19 @@ def main():
         d[f"conv2dr_n{n}_k"] = K
         d[f"conv2dr_n{n}_out"] = np.fft.irfft2(np.fft.rfft2(x.astype(np.float64)) * np.conj(K.astype(np.complex128)),
                                                s=(n, n))
+    # prime lengths (Bluestein: m = 256 and m = 16384, the latter as tiles)
+    for n, rows in ((97, 3), (4099, 2)):
+        x = crandn(rng, (rows, n)).astype(np.complex64)
+        d[f"prime_n{n}_x"] = x
+        d[f"prime_n{n}_fwd"] = np.fft.fft(x.astype(np.complex128), axis=-1)
+        d[f"prime_n{n}_inv"] = np.fft.ifft(x.astype(np.complex128), axis=-1)
+    # a ragged batch: transforms of mixed lengths packed back to back
+    sizes = np.array([64, 97, 1000, 4099, 8192])
+    x = crandn(rng, (int(sizes.sum()),)).astype(np.complex64)
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    d["ragged_sizes"] = sizes
+    d["ragged_x"] = x
+    d["ragged_fwd"] = np.concatenate([np.fft.fft(x[o:o + n].astype(np.complex128)) for o, n in zip(offs, sizes)])
     d = {k: (v.astype(np.complex64) if np.iscomplexobj(v) else v.astype(np.float32) if v.dtype == np.float64 else v)
          for k, v in d.items()}
     np.savez_compressed(OUT, **d)

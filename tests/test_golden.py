@@ -47,6 +47,21 @@ def test_oracle_signal_range_and_convolutions(g):
         assert O.rel_l2(got, g[f"conv2dr_n{n}_out"]) <= FIXTURE_REL
 
 
+@pytest.mark.parametrize("n", [97, 4099])
+def test_oracle_primes(g, n):
+    x = g[f"prime_n{n}_x"]
+    assert O.rel_l2(O.dft(x), g[f"prime_n{n}_fwd"]) <= FIXTURE_REL
+    assert O.rel_l2(O.idft(x), g[f"prime_n{n}_inv"]) <= FIXTURE_REL
+    assert O.rel_l2(O.bluestein_dft(x.astype(np.complex128)), g[f"prime_n{n}_fwd"]) <= FIXTURE_REL
+
+
+def test_oracle_ragged_batch(g):
+    sizes, x = g["ragged_sizes"], g["ragged_x"]
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    out = np.concatenate([O.dft(x[o:o + n]) for o, n in zip(offs, sizes)])
+    assert O.rel_l2(out, g["ragged_fwd"]) <= FIXTURE_REL
+
+
 # ---------------------------------------------------------------- HIP path (GPU)
 GPU_REL = 5e-6
 
@@ -100,3 +115,31 @@ def test_gpu_signal_range_and_convolutions(g):
         kb.write_fourier(g[f"conv2dr_n{n}_k"])
         vd.fft.convolve2DR(rb, kb, kernel_inner_only=True)
         assert O.rel_l2(rb.read_real(0), g[f"conv2dr_n{n}_out"]) <= 1e-5
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", [97, 4099])
+def test_gpu_primes(g, n):
+    """Bluestein through vd.fft (m = 256 whole; m = 16384 as tiles)."""
+    import vkdispatch_amd as vd
+    x = g[f"prime_n{n}_x"]
+    buf = vd.asbuffer(x)
+    vd.fft.fft(buf)
+    assert O.rel_l2(buf.read(0), g[f"prime_n{n}_fwd"]) <= 1e-5
+    buf = vd.asbuffer(x)
+    vd.fft.ifft(buf)
+    assert O.rel_l2(buf.read(0), g[f"prime_n{n}_inv"]) <= 1e-5
+
+
+@pytest.mark.gpu
+def test_gpu_ragged_batch(g):
+    import vkdispatch_amd as vd
+    from vkdispatch_amd.fft.ragged import RaggedFFT
+    sizes, x = [int(n) for n in g["ragged_sizes"]], g["ragged_x"]
+    src, dst = vd.asbuffer(x), vd.Buffer((x.size,), vd.complex64)
+    RaggedFFT(sizes)(dst, src)
+    out, ref = dst.read(0), g["ragged_fwd"]
+    off = 0
+    for n in sizes:
+        assert O.rel_l2(out[off:off + n], ref[off:off + n]) <= 1e-5, n
+        off += n
