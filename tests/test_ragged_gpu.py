@@ -140,3 +140,32 @@ def test_tune_is_cached_per_batch(tmp_path):
     assert not first["cached"] and len(first["results"]) == 6 * len(R.TUNE_TILE_MIN)
     again = R.tune(sizes, buf, reps=1, cache=True, path=path)
     assert again["cached"] and {k: again["best"][k] for k in R.TUNE_KEYS} == {k: first["best"][k] for k in R.TUNE_KEYS}
+
+
+def test_graph_replay_after_another_plan_needs_more_scratch():
+    """A captured plan launches with its own scratch: a later plan with larger
+    per-workgroup scratch does not move it, and replays stay correct."""
+    sizes = [65536, 100, 8192]
+    parts, packed = _packed(sizes, 12)
+    src = vd.asbuffer(packed)
+    dst = vd.Buffer((packed.size,), vd.complex64)
+    plan_a = R.RaggedFFT(sizes, four_step=False)  # whole transforms: per-workgroup scratch
+    g = vd.CommandGraph()
+    g.record_callable(plan_a.bind(dst, src))
+    g.capture()
+    g.submit()
+    vd.synchronize()
+    _check(sizes, parts, dst.read(0))
+    big = [1 << 18]
+    bparts, bpacked = _packed(big, 13)
+    bbuf = vd.asbuffer(bpacked)
+    plan_b = R.RaggedFFT(big, four_step=False)
+    assert plan_b.scratch_per_wg > plan_a.scratch_per_wg
+    plan_b(bbuf)
+    vd.synchronize()
+    _check(big, bparts, bbuf.read(0))
+    dst.write(np.zeros_like(packed))
+    for _ in range(2):
+        g.submit()
+        vd.synchronize()
+        _check(sizes, parts, dst.read(0))

@@ -246,14 +246,18 @@ _arena = _Arena()
 
 
 def _launch_tasks(tasks: np.ndarray, scratch_elems_per_wg: int, grid: int, block: int, stream=None,
-                  task_buf=None, counters: Optional[Tuple[int, int]] = None, spin_limit: int = 0) -> None:
+                  task_buf=None, counters: Optional[Tuple[int, int]] = None, spin_limit: int = 0,
+                  scratch: int = 0) -> None:
+    """Launch a task table; ``scratch`` (device pointer, scratch_elems_per_wg complex64
+    per workgroup) is the caller's, else the shared arena, which only synchronous
+    one-off launches use (it may be reallocated by the next such launch)."""
     s = rt.current_stream() if stream is None else int(stream)
     if task_buf is None:
         task_ptr = rt.malloc(tasks.nbytes)
         rt.htod(task_ptr, tasks.ctypes.data, tasks.nbytes, s)
     else:
         task_ptr = task_buf
-    scr = _arena.ensure(max(1, scratch_elems_per_wg) * 8 * grid)
+    scr = scratch or _arena.ensure(max(1, scratch_elems_per_wg) * 8 * grid)
     ctr, nctr = counters if counters is not None else (_arena.counter_ptr(), 2)
     rc = rt.lib().vkx_ragged_exec(task_ptr, len(tasks), ctr, nctr, scr, scratch_elems_per_wg, grid, block,
                                   int(spin_limit), s)
@@ -440,6 +444,7 @@ class RaggedFFT:
         self.grid = max(1, min(int(grid) if grid else 2 * int(dev.compute_units), len(self._items)))
         self._scr = rt.malloc(max(1, scr_elems) * 8)
         self._ctr = rt.malloc(4 * self._nctr)
+        self._wg_scr = 0
         self._task_cache: Dict[Tuple[int, int], int] = {}
 
     @property
@@ -459,9 +464,10 @@ class RaggedFFT:
         return self._task_cache[key]
 
     def bind(self, out_buf, in_buf=None):
-        """Upload the item table for this (input, output) pair and size the scratch
-        arena; returns ``launch(stream)`` that allocates nothing (safe under graph capture).
-        Launches of one plan share its counters and scratch: they must not overlap."""
+        """Upload the item table for this (input, output) pair and allocate the plan's
+        scratch; returns ``launch(stream)`` that allocates nothing (safe under graph
+        capture: every pointer it launches with belongs to the plan and lives as long as
+        it).  Launches of one plan share its counters and scratch: they must not overlap."""
         if in_buf is None:
             in_buf = out_buf
         for b in (out_buf, in_buf):
@@ -470,7 +476,9 @@ class RaggedFFT:
         if any(int(n) == 1 for n in self.sizes) and in_buf is not out_buf:
             raise ValueError("out-of-place ragged batches may not contain length-1 transforms")
         tp = self._tasks_for(int(in_buf.ptr), int(out_buf.ptr))
-        _arena.ensure(max(1, self.scratch_per_wg) * 8 * self.grid)
+        if not self._wg_scr:  # per-workgroup scratch of whole transforms, owned by the plan
+            self._wg_scr = rt.malloc(max(1, self.scratch_per_wg) * 8 * self.grid)
+        wg_scr = self._wg_scr
         keep = (out_buf, in_buf)
 
         def launch(stream=None):
@@ -479,7 +487,7 @@ class RaggedFFT:
                 _launch_tasks(self._items[a:b], self.scratch_per_wg, self.grid, self.block_first if i == 0 else self.block,
                               stream,
                               task_buf=tp + a * RTASK_DTYPE.itemsize, counters=(self._ctr, self._nctr),
-                              spin_limit=self.spin_limit)
+                              spin_limit=self.spin_limit, scratch=wg_scr)
         return launch
 
     def __call__(self, out_buf, in_buf=None, stream=None) -> None:
@@ -489,7 +497,8 @@ class RaggedFFT:
 
     def __del__(self):
         try:
-            ptrs = [getattr(self, "_scr", 0), getattr(self, "_ctr", 0)] + list(getattr(self, "_task_cache", {}).values())
+            ptrs = [getattr(self, "_scr", 0), getattr(self, "_ctr", 0), getattr(self, "_wg_scr", 0)] + \
+                list(getattr(self, "_task_cache", {}).values())
             if any(ptrs):
                 rt.synchronize()
             for p in ptrs:
