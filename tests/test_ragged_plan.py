@@ -139,13 +139,18 @@ def emulate(q, sizes, x, inverse, normalize=True):
 SIZES = [64, 97, 100, 4096, 4099, 5003, 8192, 12288, 59049, 65536, 65521, 60060, 131072, 3 * 5 * 7 * 11 * 13 * 8]
 
 
-@pytest.mark.parametrize("sync", ["counter", "launch"])
+@pytest.mark.parametrize("sync,fill", [("counter", 0), ("launch", 0), ("launch", 4096)])
 @pytest.mark.parametrize("inverse", [False, True])
 @pytest.mark.parametrize("max_radix", [8, 16])
-def test_item_queue_computes_the_dft(inverse, max_radix, sync):
+def test_item_queue_computes_the_dft(inverse, max_radix, sync, fill):
     rng = np.random.default_rng(11)
     x = rng.standard_normal(sum(SIZES)) + 1j * rng.standard_normal(sum(SIZES))
-    q = R.plan_items(SIZES, inverse, True, max_radix, True, _FakeTables(), sync, tile_min=4097)
+    q = R.plan_items(SIZES, inverse, True, max_radix, True, _FakeTables(), sync, tile_min=4097, fill=fill)
+    if fill:  # the short whole transforms sit in the later launches
+        later = set(range(q.segments[1][0], len(q.items)))
+        small = [k for k, t in enumerate(q.items) if ((int(t["npass"]) >> 16) & 0xF) == R.K_WHOLE and t["n"] <= fill
+                 and t["m"] == 0]
+        assert small and set(small) <= later
     if sync == "launch":
         assert len(q.segments) == 4 and (q.items["dep"] < 0).all() and (q.items["done"] < 0).all()
     kinds = {(int(v) >> 16) & 0xF for v in q.items["npass"]}
