@@ -312,6 +312,8 @@ __device__ __forceinline__ void conv_rows_fwd_cm(const ConvArgs<C>& a, cpx<float
   const bool has_b = ib < C::NIMG;
   const u32 y0 = slice * C::RA;
   cpx<float>* S = a.scr + (u64)pair * (u64)(C::H * C::W);
+  cpx<float> twr[list_tw_regs<WC, L>::value > 0 ? list_tw_regs<WC, L>::value : 1];
+  tw_load<WC>(L{}, twr, lane, a.twf);  // in flight with the row loads
   cpx<float> v[SH::RPW][SH::NV];
 #pragma unroll
   for (int i = 0; i < SH::RPW; ++i) {
@@ -323,7 +325,7 @@ __device__ __forceinline__ void conv_rows_fwd_cm(const ConvArgs<C>& a, cpx<float
     });
   }
 #pragma unroll
-  for (int i = 0; i < SH::RPW; ++i) run_passes<WC, -1>(L{}, v[i], lds + wave * C::NP, lane, 0, a.twf);
+  for (int i = 0; i < SH::RPW; ++i) run_passes_reg<WC, -1>(L{}, v[i], lds + wave * C::NP, lane, 0, twr);
   __syncthreads();  // the transpose tile overlaps the exchange regions
 #pragma unroll
   for (int i = 0; i < SH::RPW; ++i) {
@@ -352,6 +354,15 @@ __device__ __forceinline__ void conv_cols_cm(const ConvArgs<C>& a, const cpx<flo
   using IL = typename pl_last<LI>::type;
   const u32 kx0 = slice * (u32)(C::NW * C::CB) + (u32)(wave * C::CB);
   cpx<float>* xch = lds + wave * C::NP;
+  // every transform of the task uses the same per-lane twiddles: read them once.  A
+  // twiddle table depends only on N and the radix list (the inverse passes conjugate),
+  // so when the inverse list equals the forward one both directions share one set.
+  constexpr bool SAME = __is_same(L, LI);
+  cpx<float> twr[list_tw_regs<WC, L>::value > 0 ? list_tw_regs<WC, L>::value : 1];
+  cpx<float> twi_own[SAME || list_tw_regs<WC, LI>::value == 0 ? 1 : list_tw_regs<WC, LI>::value];
+  tw_load<WC>(L{}, twr, lane, a.twf);
+  if constexpr (!SAME) tw_load<WC>(LI{}, twi_own, lane, a.twi);
+  const cpx<float>* twir = SAME ? twr : twi_own;
 #pragma unroll
   for (int cc = 0; cc < C::CB; ++cc) {
     const u32 kx = kx0 + (u32)cc;
@@ -364,11 +375,11 @@ __device__ __forceinline__ void conv_cols_cm(const ConvArgs<C>& a, const cpx<flo
       cpx<float>* col = a.scr + (u64)pair * (u64)(C::H * C::W) + kx * C::H;
       cpx<float> v[SH::NV];
       for_layout<WC, F0::R>(lane, [&](auto slot, u32 n) { v[decltype(slot)::value] = ld_c64<L2ONLY>(col + n); });
-      run_passes<WC, -1>(L{}, v, xch, lane, 0, a.twf);
+      run_passes_reg<WC, -1>(L{}, v, xch, lane, 0, twr);
 #pragma unroll
       for (int i = 0; i < SH::NV; ++i) v[i] = cmul(v[i], kv[i]);
       block_sync<WC>();
-      run_passes<WC, 1>(LI{}, v, xch, lane, 0, a.twi);
+      run_passes_reg<WC, 1>(LI{}, v, xch, lane, 0, twir);
       for_layout<WC, IL::R>(lane, [&](auto slot, u32 n) { col[n] = v[decltype(slot)::value]; });
       block_sync<WC>();
     }
@@ -388,6 +399,8 @@ __device__ __forceinline__ void conv_rows_inv_cm(const ConvArgs<C>& a, cpx<float
   const bool has_b = ib < C::NIMG;
   const u32 y0 = slice * C::RA;
   const cpx<float>* S = a.scr + (u64)pair * (u64)(C::H * C::W);
+  cpx<float> twr[list_tw_regs<WC, LI>::value > 0 ? list_tw_regs<WC, LI>::value : 1];
+  tw_load<WC>(LI{}, twr, lane, a.twi);  // in flight with the tile loads; reused by every row
   {
     cpx<float> t[IT];  // all loads in flight before the first LDS write
 #pragma unroll
@@ -412,7 +425,7 @@ __device__ __forceinline__ void conv_rows_inv_cm(const ConvArgs<C>& a, cpx<float
 #pragma unroll
   for (int i = 0; i < SH::RPW; ++i) {
     const u32 y = y0 + (u32)(wave + C::NW * i);
-    run_passes<WC, 1>(LI{}, v[i], lds + wave * C::NP, lane, 0, a.twi);
+    run_passes_reg<WC, 1>(LI{}, v[i], lds + wave * C::NP, lane, 0, twr);
     float* ra = a.x + (u64)ia * C::IMG + y * C::PITCH;
     float* rb = a.x + (u64)ib * C::IMG + y * C::PITCH;
     for_layout<WC, IL::R>(lane, [&](auto slot, u32 n) {

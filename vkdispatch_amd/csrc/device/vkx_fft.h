@@ -517,6 +517,88 @@ __device__ __forceinline__ void run_passes(PL<Ps...>, cpx<typename C::real>* v, 
   run_passes_<C, DIR, Ps...>(v, lds, t, b, tw);
 }
 
+// Register-resident twiddles.  A thread's twiddles depend only on its index t and
+// the pass list, so a kernel that runs the same transform many times per thread can
+// read them once (tw_load) and run every transform from registers (run_passes_reg).
+// Inside a loop that also stores to global memory the compiler cannot hoist the
+// table reads itself (it cannot prove the stores leave the table alone), and each
+// pass would wait on a fresh table read.  Slot q (R - 1) + (r - 1) of a pass holds
+// its leg-r twiddle for butterfly j = t + q TT, in the order pass_compute reads them.
+template <class C, class Pa> struct pass_tw_regs {
+  static constexpr int W = C::N / Pa::R;
+  static constexpr int Q = (W + C::TT - 1) / C::TT;
+  static constexpr int value = Pa::NS > 1 ? Q * (Pa::R - 1) : 0;
+};
+template <class C, class L> struct list_tw_regs;
+template <class C, class... Ps> struct list_tw_regs<C, PL<Ps...>> {
+  static constexpr int value = (0 + ... + pass_tw_regs<C, Ps>::value);
+};
+
+template <class C, int OFF, class Cur, class... Rest>
+__device__ __forceinline__ void tw_load_(cpx<typename C::real>* w, int t, const cpx<typename C::real>* tw) {
+  constexpr int R = Cur::R, NS = Cur::NS, W = C::N / R, Q = (W + C::TT - 1) / C::TT;
+  if constexpr (NS > 1) {
+    sfor<0, Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      constexpr bool full = (q + 1) * C::TT <= W;
+      const int j = t + q * C::TT;
+      const int k = j % NS;
+      sfor<1, R>([&](auto rc) {
+        constexpr int r = decltype(rc)::value;
+        cpx<typename C::real> x{0, 0};
+        if (full || j < W) x = tw[(NS - 1) + (r - 1) * NS + k];  // rows j >= W have no butterfly
+        w[OFF + q * (R - 1) + (r - 1)] = x;
+      });
+    });
+  }
+  if constexpr (sizeof...(Rest) > 0) tw_load_<C, OFF + pass_tw_regs<C, Cur>::value, Rest...>(w, t, tw);
+}
+template <class C, class... Ps>
+__device__ __forceinline__ void tw_load(PL<Ps...>, cpx<typename C::real>* w, int t, const cpx<typename C::real>* tw) {
+  tw_load_<C, 0, Ps...>(w, t, tw);
+}
+
+template <class C, int R, int NS, int DIR>
+__device__ __forceinline__ void pass_compute_reg(cpx<typename C::real>* v, int t, const cpx<typename C::real>* w) {
+  constexpr int W = C::N / R;
+  constexpr int Q = (W + C::TT - 1) / C::TT;
+  sfor<0, Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    constexpr bool full = (q + 1) * C::TT <= W;
+    const int j = t + q * C::TT;
+    if (full || j < W) {
+      if constexpr (NS > 1) {
+        sfor<1, R>([&](auto rc) {
+          constexpr int r = decltype(rc)::value;
+          cpx<typename C::real> x = w[q * (R - 1) + (r - 1)];
+          if constexpr (DIR > 0) x.y = -x.y;
+          v[q * R + r] = cmul(v[q * R + r], x);
+        });
+      }
+      DFT<R, DIR>::run(v + q * R);
+    }
+  });
+}
+
+template <class C, int DIR, int OFF, class Cur, class... Rest>
+__device__ __forceinline__ void run_passes_reg_(cpx<typename C::real>* v, cpx<typename C::real>* lds, int t, int b,
+                                                const cpx<typename C::real>* w) {
+  pass_compute_reg<C, Cur::R, Cur::NS, DIR>(v, t, w + OFF);
+  if constexpr (sizeof...(Rest) > 0) {
+    using Nx = typename pl_first<PL<Rest...>>::type;
+    lds_put<C, Cur::R, Cur::NS>(lds, v, t, b);
+    block_sync<C>();
+    lds_get<C, Nx::R>(lds, v, t, b);
+    block_sync<C>();
+    run_passes_reg_<C, DIR, OFF + pass_tw_regs<C, Cur>::value, Rest...>(v, lds, t, b, w);
+  }
+}
+template <class C, int DIR, class... Ps>
+__device__ __forceinline__ void run_passes_reg(PL<Ps...>, cpx<typename C::real>* v, cpx<typename C::real>* lds, int t,
+                                               int b, const cpx<typename C::real>* w) {
+  run_passes_reg_<C, DIR, 0, Ps...>(v, lds, t, b, w);
+}
+
 // Iterate the register slots of the layout produced by pass (R): slot q*R+r
 // holds element n = j + r*N/R, j = t + q*TT (first pass input / last pass output).
 template <class C, int R, class F>
