@@ -50,6 +50,12 @@ template <int N_, int NP_> struct WaveCfg {
   static constexpr bool COLFAST = false, WAVE_LOCAL = true;
 };
 
+// Register-resident twiddles (loaded once per task, see TwSet) when a lane's set is
+// small: 14 values for 512 = 8.8.8 and 9 for 256 = 4.4.4.4.  The 27 of 1024 =
+// 16.16.4 would push the cooperative 1024 kernel into spills, so those kernels keep
+// reading the table.
+template <class WC, class L> constexpr bool conv_twreg() { return list_tw_regs<WC, L>::value <= 16; }
+
 // control block layout (u32 words)
 constexpr u32 CTL_TICKET = 0;      // + 32 q
 constexpr u32 CTL_EXIT = 256;      // workgroups that have left
@@ -312,8 +318,8 @@ __device__ __forceinline__ void conv_rows_fwd_cm(const ConvArgs<C>& a, cpx<float
   const bool has_b = ib < C::NIMG;
   const u32 y0 = slice * C::RA;
   cpx<float>* S = a.scr + (u64)pair * (u64)(C::H * C::W);
-  cpx<float> twr[list_tw_regs<WC, L>::value > 0 ? list_tw_regs<WC, L>::value : 1];
-  tw_load<WC>(L{}, twr, lane, a.twf);  // in flight with the row loads
+  TwSet<WC, L, conv_twreg<WC, L>()> ts;
+  ts.load(lane, a.twf);  // in flight with the row loads
   cpx<float> v[SH::RPW][SH::NV];
 #pragma unroll
   for (int i = 0; i < SH::RPW; ++i) {
@@ -325,7 +331,7 @@ __device__ __forceinline__ void conv_rows_fwd_cm(const ConvArgs<C>& a, cpx<float
     });
   }
 #pragma unroll
-  for (int i = 0; i < SH::RPW; ++i) run_passes_reg<WC, -1>(L{}, v[i], lds + wave * C::NP, lane, 0, twr);
+  for (int i = 0; i < SH::RPW; ++i) ts.template run<-1>(v[i], lds + wave * C::NP, lane, 0);
   __syncthreads();  // the transpose tile overlaps the exchange regions
 #pragma unroll
   for (int i = 0; i < SH::RPW; ++i) {
@@ -358,11 +364,10 @@ __device__ __forceinline__ void conv_cols_cm(const ConvArgs<C>& a, const cpx<flo
   // twiddle table depends only on N and the radix list (the inverse passes conjugate),
   // so when the inverse list equals the forward one both directions share one set.
   constexpr bool SAME = __is_same(L, LI);
-  cpx<float> twr[list_tw_regs<WC, L>::value > 0 ? list_tw_regs<WC, L>::value : 1];
-  cpx<float> twi_own[SAME || list_tw_regs<WC, LI>::value == 0 ? 1 : list_tw_regs<WC, LI>::value];
-  tw_load<WC>(L{}, twr, lane, a.twf);
-  if constexpr (!SAME) tw_load<WC>(LI{}, twi_own, lane, a.twi);
-  const cpx<float>* twir = SAME ? twr : twi_own;
+  TwSet<WC, L, conv_twreg<WC, L>()> tsf;
+  TwSet<WC, LI, SAME ? false : conv_twreg<WC, LI>()> tsi_own;
+  tsf.load(lane, a.twf);
+  tsi_own.load(lane, a.twi);
 #pragma unroll
   for (int cc = 0; cc < C::CB; ++cc) {
     const u32 kx = kx0 + (u32)cc;
@@ -375,11 +380,12 @@ __device__ __forceinline__ void conv_cols_cm(const ConvArgs<C>& a, const cpx<flo
       cpx<float>* col = a.scr + (u64)pair * (u64)(C::H * C::W) + kx * C::H;
       cpx<float> v[SH::NV];
       for_layout<WC, F0::R>(lane, [&](auto slot, u32 n) { v[decltype(slot)::value] = ld_c64<L2ONLY>(col + n); });
-      run_passes_reg<WC, -1>(L{}, v, xch, lane, 0, twr);
+      tsf.template run<-1>(v, xch, lane, 0);
 #pragma unroll
       for (int i = 0; i < SH::NV; ++i) v[i] = cmul(v[i], kv[i]);
       block_sync<WC>();
-      run_passes_reg<WC, 1>(LI{}, v, xch, lane, 0, twir);
+      if constexpr (SAME) tsf.template run<1>(v, xch, lane, 0);
+      else tsi_own.template run<1>(v, xch, lane, 0);
       for_layout<WC, IL::R>(lane, [&](auto slot, u32 n) { col[n] = v[decltype(slot)::value]; });
       block_sync<WC>();
     }
@@ -399,8 +405,8 @@ __device__ __forceinline__ void conv_rows_inv_cm(const ConvArgs<C>& a, cpx<float
   const bool has_b = ib < C::NIMG;
   const u32 y0 = slice * C::RA;
   const cpx<float>* S = a.scr + (u64)pair * (u64)(C::H * C::W);
-  cpx<float> twr[list_tw_regs<WC, LI>::value > 0 ? list_tw_regs<WC, LI>::value : 1];
-  tw_load<WC>(LI{}, twr, lane, a.twi);  // in flight with the tile loads; reused by every row
+  TwSet<WC, LI, conv_twreg<WC, LI>()> ts;
+  ts.load(lane, a.twi);  // in flight with the tile loads; reused by every row
   {
     cpx<float> t[IT];  // all loads in flight before the first LDS write
 #pragma unroll
@@ -425,7 +431,7 @@ __device__ __forceinline__ void conv_rows_inv_cm(const ConvArgs<C>& a, cpx<float
 #pragma unroll
   for (int i = 0; i < SH::RPW; ++i) {
     const u32 y = y0 + (u32)(wave + C::NW * i);
-    run_passes_reg<WC, 1>(LI{}, v[i], lds + wave * C::NP, lane, 0, twr);
+    ts.template run<1>(v[i], lds + wave * C::NP, lane, 0);
     float* ra = a.x + (u64)ia * C::IMG + y * C::PITCH;
     float* rb = a.x + (u64)ib * C::IMG + y * C::PITCH;
     for_layout<WC, IL::R>(lane, [&](auto slot, u32 n) {

@@ -599,6 +599,31 @@ __device__ __forceinline__ void run_passes_reg(PL<Ps...>, cpx<typename C::real>*
   run_passes_reg_<C, DIR, 0, Ps...>(v, lds, t, b, w);
 }
 
+// C::TWREG (a plan option; false when the configuration does not name it): the
+// plan's kernels take their twiddles from registers, loaded at the start of the
+// kernel together with the data, instead of from the table after each exchange.
+// More registers, fewer exposed waits; the autotuner decides per plan.
+template <class C, class = void> struct twreg_of { static constexpr bool value = false; };
+template <class C> struct twreg_of<C, decltype((void)C::TWREG)> { static constexpr bool value = C::TWREG; };
+
+template <class C, class L, bool REG = twreg_of<C>::value> struct TwSet {
+  const cpx<typename C::real>* tw;
+  __device__ __forceinline__ void load(int, const cpx<typename C::real>* table) { tw = table; }
+  template <int DIR>
+  __device__ __forceinline__ void run(cpx<typename C::real>* v, cpx<typename C::real>* lds, int t, int b) const {
+    run_passes<C, DIR>(L{}, v, lds, t, b, tw);
+  }
+};
+template <class C, class L> struct TwSet<C, L, true> {
+  static constexpr int NW = list_tw_regs<C, L>::value;
+  cpx<typename C::real> w[NW > 0 ? NW : 1];
+  __device__ __forceinline__ void load(int t, const cpx<typename C::real>* table) { tw_load<C>(L{}, w, t, table); }
+  template <int DIR>
+  __device__ __forceinline__ void run(cpx<typename C::real>* v, cpx<typename C::real>* lds, int t, int b) const {
+    run_passes_reg<C, DIR>(L{}, v, lds, t, b, w);
+  }
+};
+
 // Iterate the register slots of the layout produced by pass (R): slot q*R+r
 // holds element n = j + r*N/R, j = t + q*TT (first pass input / last pass output).
 template <class C, int R, class F>
@@ -644,8 +669,10 @@ __device__ __forceinline__ void fft_body(const KParams& prm) {
   } else {
     sfor<0, C::VMAX>([&](auto s) { v[decltype(s)::value] = cpx<real>{(real)0, (real)0}; });
   }
+  TwSet<C, L> ts;
+  ts.load(g.t, tw);
 
-  run_passes<C, (C::INV ? 1 : -1)>(L{}, v, lds, g.t, g.b, tw);
+  ts.template run<(C::INV ? 1 : -1)>(v, lds, g.t, g.b);
 
   if (g.valid) {
     const u32 boff = out_boff<C>(g);
@@ -706,8 +733,15 @@ __device__ __forceinline__ void conv_body(const KParams& prm) {
   } else {
     sfor<0, C::VMAX>([&](auto s) { v[decltype(s)::value] = cpx<real>{(real)0, (real)0}; });
   }
+  // a twiddle table depends only on N and the radix list (the inverse passes
+  // conjugate): a palindromic radix list shares one set between the directions
+  constexpr bool SAME = __is_same(L, LI);
+  TwSet<C, L> tsf;
+  tsf.load(g.t, twf);
+  TwSet<C, LI, SAME ? false : twreg_of<C>::value> tsi_own;
+  tsi_own.load(g.t, twi);
 
-  run_passes<C, -1>(L{}, v, lds, g.t, g.b, twf);
+  tsf.template run<-1>(v, lds, g.t, g.b);
 
   cpx<real> bak[C::KNUM > 1 ? C::VMAX : 1];
   if constexpr (C::KNUM > 1) {
@@ -728,7 +762,8 @@ __device__ __forceinline__ void conv_body(const KParams& prm) {
         KMap::template apply<kidx>(op, prm);
       });
     }
-    run_passes<C, 1>(LI{}, v, lds, g.t, g.b, twi);
+    if constexpr (SAME) tsf.template run<1>(v, lds, g.t, g.b);
+    else tsi_own.template run<1>(v, lds, g.t, g.b);
     if (g.valid) {
       const u32 boff = out_boff<C>(g);
       const real scale = (real)C::SCALE;
@@ -791,8 +826,10 @@ __device__ __forceinline__ void rfft_body(const KParams& prm) {
   } else {
     sfor<0, C::VMAX>([&](auto s) { v[decltype(s)::value] = cpx<real>{(real)0, (real)0}; });
   }
+  TwSet<C, L> ts;
+  ts.load(g.t, tw);
 
-  run_passes<C, -1>(L{}, v, lds, g.t, g.b, tw);
+  ts.template run<-1>(v, lds, g.t, g.b);
 
   // natural-order spectrum into LDS, then Hermitian post-processing in pairs (k, M-k)
   for_layout<C, FL::R>(g.t, [&](auto slot, u32 n) { lds[lds_addr<C>(g.b, (int)n)] = v[decltype(slot)::value]; });
@@ -829,6 +866,8 @@ __device__ __forceinline__ void irfft_body(const KParams& prm) {
   const cpx<real>* roots = bufp<const cpx<real>>(prm, SLOT_ROOTS);
   const cpx<S>* Xin = bufp<const cpx<S>>(prm, OOP ? 1 : 0);
   S* xout = bufp<S>(prm, 0);
+  TwSet<C, L> ts;
+  ts.load(g.t, tw);
 
   if (g.valid) {
     const u32 ib = g.outer * C::IN_OS;
@@ -845,7 +884,7 @@ __device__ __forceinline__ void irfft_body(const KParams& prm) {
     sfor<0, C::VMAX>([&](auto s) { v[decltype(s)::value] = cpx<real>{(real)0, (real)0}; });
   }
 
-  run_passes<C, 1>(L{}, v, lds, g.t, g.b, tw);
+  ts.template run<1>(v, lds, g.t, g.b);
   block_sync<C>();  // every load of the row precedes every (in-place) store
 
   if (g.valid) {

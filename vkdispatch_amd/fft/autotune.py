@@ -89,14 +89,15 @@ def save_table(table: Dict, path: str = TABLE_PATH) -> None:
 
 def _build(kind: str, shape, axis, geom, real: str, cand):
     from . import shader_factories as SF
-    rads, tt, B = cand
+    rads, tt, B = cand[:3]
+    twreg = bool(cand[3]) if len(cand) > 3 else False
     c64 = dt.complex128 if real == "double" else dt.complex64
     if kind in ("rfft", "irfft"):
         M = geom.N // 2
         g = P.AxisGeometry((geom.OUTER, M), 1, M, 1, geom.OUTER)
     else:
         g = geom
-    P.set_override(g, real, kind, rads, tt, B)
+    P.set_override(g, real, kind, rads, tt, B, twreg)
     try:
         SF.cache_clear()
         if kind == "conv":
@@ -121,14 +122,16 @@ def _tuning_geometry(kind, shape, axis):
 
 
 def tune(kind: str, shape, axis=None, real: str = "float", cands=None, reps: int = 20,
-         compile_only: bool = False, table: Optional[Dict] = None) -> Dict:
+         compile_only: bool = False, table: Optional[Dict] = None, twreg=(False, True)) -> Dict:
     """Time every candidate for (kind, shape, axis) and record the fastest in ``table``
-    (default: the committed table).  Returns {"key", "best", "results"}."""
+    (default: the committed table).  A candidate is (radices, threads per transform,
+    transforms per workgroup[, twiddles in registers]); the default set takes every
+    plan of ``candidates`` with each ``twreg`` choice.  Returns {"key", "best", "results"}."""
     from . import kernels as K
     geom, tgeom = _tuning_geometry(kind, shape, axis)
     elem = 16 if real == "double" else 8
     if cands is None:
-        cands = candidates(tgeom.N, tgeom.S > 1, elem)
+        cands = [c + (bool(tw),) for c in candidates(tgeom.N, tgeom.S > 1, elem) for tw in twreg]
     built = []
     for c in cands:
         try:
@@ -162,11 +165,13 @@ def tune(kind: str, shape, axis=None, real: str = "float", cands=None, reps: int
     best_ms, best = min(results, key=lambda r: r[0])
     if table is None:
         table = load_table()
-    table.setdefault("plans", {})[key] = {"radices": list(best[0]), "tt": best[1], "B": best[2]}
+    table.setdefault("plans", {})[key] = {"radices": list(best[0]), "tt": best[1], "B": best[2],
+                                          "twreg": bool(best[3]) if len(best) > 3 else False}
     table.setdefault("measurements", {})[key] = {"us": round(best_ms * 1e3, 3), "shape": list(shape),
                                                  "axis": axis, "candidates": len(results)}
     return {"key": key, "best": best, "best_us": best_ms * 1e3,
-            "results": sorted([(round(ms * 1e3, 3), list(c[0]), c[1], c[2]) for ms, c in results])}
+            "results": sorted([(round(ms * 1e3, 3), list(c[0]), c[1], c[2], bool(c[3]) if len(c) > 3 else False)
+                               for ms, c in results])}
 
 
 # Shapes whose plans the committed table covers.

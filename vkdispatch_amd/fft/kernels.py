@@ -118,7 +118,8 @@ def cfg_struct(plan: KernelPlan, *, in_os: int, out_os: int, k_os: int, sig: Tup
         "struct Cfg {",
         f"  using real = {plan.real};",
         f"  static constexpr int N = {plan.N}, TT = {plan.TT}, B = {plan.B}, BLOCK = {plan.BLOCK}, VMAX = {plan.VMAX};",
-        f"  static constexpr bool COLFAST = {b(plan.COLFAST)}, WAVE_LOCAL = {b(plan.WAVE_LOCAL)};",
+        f"  static constexpr bool COLFAST = {b(plan.COLFAST)}, WAVE_LOCAL = {b(plan.WAVE_LOCAL)}, "
+        f"TWREG = {b(plan.TWREG)};",
         f"  static constexpr vkx::u32 S = {plan.S}u, OUTER = {plan.OUTER}u, NWG_INNER = {plan.NWG_INNER}u;",
         f"  static constexpr vkx::u32 IN_OS = {in_os}u, OUT_OS = {out_os}u, K_OS = {k_os}u;",
         f"  static constexpr int PAD_SHIFT = {plan.PAD_SHIFT}, NP = {plan.NP}, LDS_ELEMS = {plan.LDS_ELEMS};",

@@ -124,6 +124,7 @@ class KernelPlan:
     LDS_ELEMS: int
     NWG: int
     NWG_INNER: int
+    TWREG: bool = False  # twiddles in registers, loaded with the data (a tuned option)
 
     @property
     def BLOCK(self) -> int:
@@ -163,14 +164,15 @@ def plan_key(geom: AxisGeometry, real: str, kind: str) -> str:
     return f"{kind}|{geom.N}|{geom.S}|{real}"
 
 
-def set_override(geom: AxisGeometry, real: str, kind: str, radices=None, tt=None, B=None) -> None:
+def set_override(geom: AxisGeometry, real: str, kind: str, radices=None, tt=None, B=None, twreg=False) -> None:
     """Force a plan (used by the autotuner); ``radices=None`` removes the override.
     Kernel factories are cached: call ``vd.fft.cache_clear()`` after changing overrides."""
     key = plan_key(geom, real, kind)
     if radices is None:
         _OVERRIDES.pop(key, None)
     else:
-        _OVERRIDES[key] = {"radices": list(radices), "tt": int(tt), "B": None if B is None else int(B)}
+        _OVERRIDES[key] = {"radices": list(radices), "tt": int(tt), "B": None if B is None else int(B),
+                           "twreg": bool(twreg)}
 
 
 def _tuned_table() -> dict:
@@ -201,11 +203,13 @@ def make_plan(geom: AxisGeometry, real: str = "float", batch_hint: Optional[int]
     """Plan for ``geom``.  ``need_lds``: the kernel exchanges a full transform through
     LDS after its passes (real-FFT post-processing), so reserve B padded rows.
     ``kind`` selects the tuned-plan entry (autotuner results / overrides) if any."""
+    twreg = False
     if radices is None:
         hit = lookup(geom, real, kind)
         if hit is not None:
             radices, tt = hit["radices"], hit["tt"]
             B_force = hit.get("B") or B_force
+            twreg = bool(hit.get("twreg", False))
     N, S, OUTER = geom.N, geom.S, geom.OUTER
     elem = 16 if real == "double" else 8
     strided = S > 1
@@ -269,7 +273,7 @@ def make_plan(geom: AxisGeometry, real: str = "float", batch_hint: Optional[int]
         raise ValueError("launch grid too large")
     return KernelPlan(real=real, N=N, TT=tt, B=B, S=S, OUTER=OUTER, radices=tuple(radices), COLFAST=colfast,
                       WAVE_LOCAL=wave_local, PAD_SHIFT=pad_shift, NP=NP, LDS_ELEMS=lds_elems, NWG=nwg,
-                      NWG_INNER=nwg_inner)
+                      NWG_INNER=nwg_inner, TWREG=twreg)
 
 
 def transposed_size(plan: KernelPlan) -> int:
