@@ -265,3 +265,28 @@ def test_map_resolves_postponed_annotations():
     m = vd.map(ns["hook"])
     import vkdispatch_amd.codegen as vc
     assert m.buffer_types == [vc.Buffer[vd.complex64], vc.Var[vd.float32]]
+
+
+def test_tuned_plans_do_not_spill():
+    """Every plan of the committed tuned table compiles without register spills (the
+    tuner refuses spilling candidates; a spilling kernel has given wrong results on
+    the MI355X), and every table entry belongs to a shape listed here."""
+    from vkdispatch_amd.fft import autotune as AT
+    from vkdispatch_amd.fft import kernels as K
+    from vkdispatch_amd.fft import planner as P
+    plans = AT.load_table().get("plans", {})
+    jobs = list(AT.STANDARD_JOBS) + [("fft", (32, 1024, 513), 1)]
+    seen, built = set(), []
+    for kind, shape, axis in jobs:
+        geom, tgeom = AT._tuning_geometry(kind, shape, axis)
+        key = P.plan_key(tgeom, "float", kind)
+        if key not in plans or key in seen:
+            continue
+        seen.add(key)
+        e = plans[key]
+        cand = (tuple(e["radices"]), e["tt"], e.get("B"), bool(e.get("twreg", False)))
+        built.append((key, AT._build(kind, shape, axis, geom, "float", cand)))
+    assert seen == set(plans), f"tuned entries without a shape here: {sorted(set(plans) - seen)}"
+    K.precompile([k for _, k in built])
+    for key, k in built:
+        assert k.code_object_info()["private_bytes"] == 0, (key, k.code_object_info())

@@ -139,6 +139,9 @@ def tune(kind: str, shape, axis=None, real: str = "float", cands=None, reps: int
         except ValueError:
             continue
     K.precompile([k for _, k in built])
+    # a plan that spills registers to scratch is never picked (a spilling kernel
+    # has produced wrong results on this part)
+    built = [(c, k) for c, k in built if k.code_object_info()["private_bytes"] == 0]
     key = P.plan_key(tgeom, real, kind)
     if compile_only:
         return {"key": key, "best": None, "results": [], "compiled": len(built)}
