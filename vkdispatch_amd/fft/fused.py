@@ -215,7 +215,8 @@ class Conv2DPhases:
 
     def __init__(self, batch: int, H: int, W: int, normalize: bool = True, waves: int = 8, rows: int = 16,
                  col_waves: int = 8, layout: str = "column", cols_per_wave: int = 1, pairs_per_task: int = 4,
-                 waves_per_simd: Tuple[Optional[int], Optional[int], Optional[int]] = (None, None, None)):
+                 waves_per_simd: Tuple[Optional[int], Optional[int], Optional[int]] = (None, None, None),
+                 irow_waves: Optional[int] = None):
         self.batch, self.H, self.W = int(batch), int(H), int(W)
         self.npairs = (self.batch + 1) // 2
         self.layout = layout
@@ -227,7 +228,7 @@ class Conv2DPhases:
         npg = (self.npairs + pp - 1) // pp
         nk = (H // KPREP_BLOCK) * (W // KPREP_BLOCK)
         for phase, tag in enumerate(("rows", "cols", "irows")):
-            nw = col_waves if phase == 1 else waves
+            nw = col_waves if phase == 1 else (irow_waves or waves) if phase == 2 else waves
             cfg, plan = _conv_cfg(batch, H, W, normalize, nw, rows if phase != 1 else 16, lay, cols_per_wave, pp)
             nwg = (self.npairs * na + nk, npg * nb, self.npairs * na)[phase]
             plan = KernelPlan(**{**plan.__dict__, "NWG": nwg})
@@ -279,13 +280,16 @@ def _mode() -> str:
 
 # cooperative-kernel tilings that keep every value in registers (no private memory)
 _FUSED_TILING = {1024: dict(waves=4, rows=8)}
+# phase-kernel tilings: for 512 the row kernels run one row per wave (16 waves, 64 VGPRs,
+# no spills), so two workgroups per CU hold 8 waves per SIMD while they read HBM
+_PHASE_TILING = {512: dict(waves=16, irow_waves=8)}
 
 
 @lru_cache(maxsize=32)
 def fused_plan(batch: int, H: int, W: int, normalize: bool = True, mode: str = "phases"):
     if mode == "fused":
         return Conv2DFused(batch, H, W, normalize, **_FUSED_TILING.get(W, {}))
-    return Conv2DPhases(batch, H, W, normalize)
+    return Conv2DPhases(batch, H, W, normalize, **_PHASE_TILING.get(W, {}))
 
 
 def fused_eligible(buffer, kernel, kernel_map=None, buffer_shape=None, transposed_kernel=False,
