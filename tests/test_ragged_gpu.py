@@ -124,6 +124,8 @@ def test_ragged_wait_gives_up_and_the_launch_ends():
     assert waits.size > 0
     plan._items["target"][waits[0]] += 1000
     plan(buf)
+    with pytest.raises(RuntimeError, match="gave up"):
+        vd.synchronize()  # the deferred check reports the abandoned wait
     assert plan.aborted()
     ok = R.RaggedFFT(sizes, spin_limit=1000, sync="counter", tile_min=4097)
     buf.write(packed)

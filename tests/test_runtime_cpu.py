@@ -1,0 +1,123 @@
+"""CPU-only tests of the runtime's host logic: C-ABI error plumbing through ctypes
+(every nonzero return carries a message), the hiprtc compile step's failure path
+(bad source -> nonzero return, compiler log, error string; nothing launched), and
+the CommandGraph's ownership of the buffers it recorded."""
+import ctypes
+import gc
+import weakref
+
+import pytest
+
+from vkdispatch_amd import runtime as rt
+from vkdispatch_amd.graph import CommandGraph
+
+
+def _err():
+    msg = rt.lib().vkx_last_error().decode()
+    rt.lib().vkx_clear_error()
+    return msg
+
+
+@pytest.mark.parametrize("args,needle", [
+    ((0, 4, 0, 2, 0, 0, 1, 63, 0, 0), "block"),     # block not a multiple of 64
+    ((0, 4, 0, 2, 0, 0, 1, 1024, 0, 0), "block"),   # block above 512
+    ((0, 4, 0, 2, 0, 0, 0, 256, 0, 0), "grid"),     # no workgroups
+    ((0, 4, 0, 1, 0, 0, 1, 256, 0, 0), "counters"),  # no abort-flag slot
+    ((0, 4, 0, 2, 0, 0, 1, 256, 0, 0), "null"),     # null task array
+])
+def test_ragged_exec_rejects_bad_arguments_with_a_message(args, needle):
+    rt.lib().vkx_clear_error()
+    rc = rt.lib().vkx_ragged_exec(*args)
+    assert rc == 1
+    msg = _err()
+    assert msg.startswith("vkx_ragged_exec:") and needle in msg, msg
+
+
+def test_ragged_exec_with_no_tasks_is_a_no_op():
+    assert rt.lib().vkx_ragged_exec(0, 0, 0, 0, 0, 0, 0, 0, 0, 0) == 0
+    assert _err() == ""
+
+
+@pytest.mark.parametrize("n,m,needle", [(0, 8, "n must"), (5, 8, "m must"), (4, 7, "null")])
+def test_bluestein_b_rejects_bad_arguments_with_a_message(n, m, needle):
+    rt.lib().vkx_clear_error()
+    rc = rt.lib().vkx_bluestein_b(0, n, m, 0)
+    assert rc == 1
+    msg = _err()
+    assert msg.startswith("vkx_bluestein_b:") and needle in msg, msg
+
+
+def test_plan_create_and_exec_reject_null_handles():
+    d = rt.PlanDesc()
+    h = ctypes.c_uint64(123)
+    assert rt.lib().vkx_plan_create(ctypes.byref(h), ctypes.byref(d)) == 1
+    assert h.value == 0  # 0 = no handle, the reference's convention
+    assert "null function" in _err()
+    assert rt.lib().vkx_plan_exec(0, None, 0, None, 0, 0) == 1
+    assert "null plan" in _err()
+
+
+def _compile(source: str, opts=("--offload-arch=gfx950",)):
+    arr = (ctypes.c_char_p * len(opts))(*[o.encode() for o in opts])
+    code, size, log = ctypes.c_void_p(), ctypes.c_size_t(), ctypes.c_void_p()
+    rc = rt.lib().vkx_compile(source.encode(), b"t.hip", arr, len(opts), ctypes.byref(code), ctypes.byref(size),
+                              ctypes.byref(log))
+    log_text = ""
+    if log.value:
+        log_text = ctypes.string_at(log.value).decode(errors="replace")
+        rt.lib().vkx_free_host(log)
+    if code.value:
+        rt.lib().vkx_free_host(code)
+    return rc, size.value, log_text, code.value
+
+
+def test_compile_error_returns_nonzero_with_log_and_message():
+    """The reference's stage_compute_plan_create reports a failed build with the source and
+    the compiler log (vkdispatch/shader/shader_function.py:147-160); here vkx_compile
+    returns 1, fills the log and sets the error string, and produces no code object."""
+    bad = 'extern "C" __global__ void k(float* p) { p[0] = undefined_name; }\n'
+    rc, size, log_text, code = _compile(bad)
+    assert rc == 1
+    assert size == 0 and not code
+    assert "undefined_name" in log_text
+    assert _err().startswith("hiprtcCompileProgram:")
+
+
+def test_compile_good_source_gives_a_code_object():
+    good = 'extern "C" __global__ void k(float* p) { p[threadIdx.x] = 1.0f; }\n'
+    rc, size, _log, _code = _compile(good)
+    assert rc == 0 and size > 1000
+    assert _err() == ""
+
+
+# ---------------------------------------------------------------------------
+# CommandGraph keeps what it recorded alive
+# ---------------------------------------------------------------------------
+class _FakeBuffer:
+    def __init__(self, ptr):
+        self.ptr = ptr
+
+
+class _FakeKernel:
+    name = "fake"
+
+    def bind(self, values):
+        return [v.ptr for v in values], b""
+
+    def handle(self):
+        return 1
+
+
+def test_graph_record_keeps_plan_scratch_alive():
+    """A plan-owned scratch buffer must survive the plan's eviction from its cache while a
+    graph that recorded it can still replay (the replay would otherwise write freed memory)."""
+    g = CommandGraph()
+    user, scratch = _FakeBuffer(0x1000), _FakeBuffer(0x2000)
+    g.record_kernel(_FakeKernel(), [user, scratch])
+    ref = weakref.ref(scratch)
+    del scratch
+    gc.collect()
+    assert ref() is not None, "the graph dropped a buffer it will replay into"
+    g.reset()
+    gc.collect()
+    assert ref() is None

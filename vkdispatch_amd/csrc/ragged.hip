@@ -21,6 +21,7 @@
 
 #include "../../include/vkdfft.h"
 #include "device/vkx_fft.h"
+#include "vkx_error.h"
 
 namespace {
 
@@ -445,23 +446,29 @@ extern "C" {
 int vkx_ragged_exec(uint64_t tasks, int ntasks, uint64_t counters, int ncounters, uint64_t scratch,
                     long long scratch_per_wg, int grid, int block, int spin_limit, uint64_t stream) {
   if (ntasks <= 0) return 0;
-  if (block < 64 || block > 512 || (block % 64) != 0 || grid < 1 || ncounters < 2) {
-    return 1;
-  }
+  if (block < 64 || block > 512 || (block % 64) != 0)
+    return vkx_set_error("vkx_ragged_exec", "block must be a multiple of 64 in [64, 512]");
+  if (grid < 1) return vkx_set_error("vkx_ragged_exec", "grid must be >= 1");
+  if (ncounters < 2) return vkx_set_error("vkx_ragged_exec", "need at least 2 counters (queue position, abort flag)");
+  if (!tasks || !counters) return vkx_set_error("vkx_ragged_exec", "null task or counter array");
   if (spin_limit <= 0) spin_limit = 1 << 20;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   hipLaunchKernelGGL(ragged_reset_kernel, dim3(1), dim3(256), 0, s, reinterpret_cast<int*>(counters), ncounters);
   hipLaunchKernelGGL(ragged_kernel, dim3(grid), dim3(block), 0, s, reinterpret_cast<const RTask*>(tasks), ntasks,
                      reinterpret_cast<int*>(counters), reinterpret_cast<c64*>(scratch), scratch_per_wg, spin_limit);
-  return hipGetLastError() == hipSuccess ? 0 : 1;
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : vkx_set_hip_error("vkx_ragged_exec: launch", e);
 }
 
 int vkx_bluestein_b(uint64_t dst, int n, int m, uint64_t stream) {
-  if (n < 1 || m < 2 * n - 1) return 1;
+  if (n < 1) return vkx_set_error("vkx_bluestein_b", "n must be >= 1");
+  if (m < 2 * n - 1) return vkx_set_error("vkx_bluestein_b", "m must be >= 2n - 1");
+  if (!dst) return vkx_set_error("vkx_bluestein_b", "null destination");
   const int threads = 256;
   hipLaunchKernelGGL(bluestein_b_kernel, dim3((m + threads - 1) / threads), dim3(threads), 0,
                      reinterpret_cast<hipStream_t>(stream), reinterpret_cast<c64*>(dst), n, m);
-  return hipGetLastError() == hipSuccess ? 0 : 1;
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : vkx_set_hip_error("vkx_bluestein_b: launch", e);
 }
 
 }  // extern "C"

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../../include/vkdfft.h"
+#include "vkx_error.h"
 
 namespace {
 
@@ -25,7 +26,7 @@ thread_local std::string g_err;
 
 int fail(const char* where, const std::string& what) {
   g_err = std::string(where) + ": " + what;
-  return 1;
+  return VKX_ERR;
 }
 
 int hip_fail(const char* where, hipError_t e) {
@@ -149,6 +150,19 @@ struct Plan {
 std::mutex g_mu;
 std::map<TwKey, uint64_t> g_tw_cache;  // device-resident twiddle tables, kept for the process
 std::map<std::pair<int, int>, uint64_t> g_roots_cache;
+hipStream_t g_table_stream = nullptr;  // the library's own stream for table generation
+
+// Runs gen(stream) on the library's private non-blocking stream and waits for that
+// stream only.  Table generation therefore never touches the null stream or a
+// caller's stream: a plan may be created while the caller's stream is being
+// captured into a graph (torch.cuda.graph, hipStreamBeginCapture), and nothing of
+// the generation is recorded into that graph.  Called with g_mu held.
+template <class F> int fill_table(uint64_t /*dst*/, F&& gen) {
+  if (!g_table_stream) VKX_HIP(hipStreamCreateWithFlags(&g_table_stream, hipStreamNonBlocking));
+  if (const int rc = gen(g_table_stream)) return rc;
+  VKX_HIP(hipStreamSynchronize(g_table_stream));
+  return 0;
+}
 
 int get_roots(uint64_t* out, int n, int dbl) {
   *out = 0;
@@ -163,8 +177,10 @@ int get_roots(uint64_t* out, int n, int dbl) {
   void* p = nullptr;
   VKX_HIP(hipMalloc(&p, (size_t)n * (dbl ? 16 : 8)));
   const uint64_t dptr = reinterpret_cast<uint64_t>(p);
-  if (vkx_roots(dptr, n, dbl, 0) != 0) return 1;
-  VKX_HIP(hipStreamSynchronize(0));
+  if (const int rc = fill_table(dptr, [&](hipStream_t s) { return vkx_roots(dptr, n, dbl, (uint64_t)s); })) {
+    (void)hipFree(p);
+    return rc;
+  }
   g_roots_cache[key] = dptr;
   *out = dptr;
   return 0;
@@ -184,14 +200,19 @@ int get_twiddles(uint64_t* out, int n, const int* radices, int npass, int dbl) {
   void* p = nullptr;
   VKX_HIP(hipMalloc(&p, bytes));
   const uint64_t dptr = reinterpret_cast<uint64_t>(p);
-  if (vkx_twiddles(dptr, n, radices, npass, dbl, 0) != 0) return 1;
-  VKX_HIP(hipStreamSynchronize(0));
+  if (const int rc = fill_table(dptr, [&](hipStream_t s) { return vkx_twiddles(dptr, n, radices, npass, dbl, (uint64_t)s); })) {
+    (void)hipFree(p);
+    return rc;
+  }
   g_tw_cache[key] = dptr;
   *out = dptr;
   return 0;
 }
 
 }  // namespace
+
+int vkx_set_error(const char* where, const char* what) { return fail(where, what); }
+int vkx_set_hip_error(const char* where, hipError_t e) { return hip_fail(where, e); }
 
 extern "C" {
 
@@ -481,13 +502,19 @@ int vkx_plan_create(uint64_t* plan, const vkx_plan_desc* d) {
   p->shared = d->shared_bytes;
   p->tw_fwd = p->tw_inv = p->roots = 0;
   if (d->roots_n > 0) {
-    if (get_roots(&p->roots, d->roots_n, d->double_precision)) { delete p; return 1; }
+    if (const int rc = get_roots(&p->roots, d->roots_n, d->double_precision)) { delete p; return rc; }
   }
   if (d->n > 1 && d->num_passes_fwd > 0) {
-    if (get_twiddles(&p->tw_fwd, d->n, d->radices_fwd, d->num_passes_fwd, d->double_precision)) { delete p; return 1; }
+    if (const int rc = get_twiddles(&p->tw_fwd, d->n, d->radices_fwd, d->num_passes_fwd, d->double_precision)) {
+      delete p;
+      return rc;
+    }
   }
   if (d->n > 1 && d->num_passes_inv > 0) {
-    if (get_twiddles(&p->tw_inv, d->n, d->radices_inv, d->num_passes_inv, d->double_precision)) { delete p; return 1; }
+    if (const int rc = get_twiddles(&p->tw_inv, d->n, d->radices_inv, d->num_passes_inv, d->double_precision)) {
+      delete p;
+      return rc;
+    }
   }
   *plan = reinterpret_cast<uint64_t>(p);
   return 0;

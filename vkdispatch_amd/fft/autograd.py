@@ -32,7 +32,7 @@ from .. import runtime as rt
 from ..buffer import Buffer
 from ..shader_map import map as vd_map
 from . import functions as F
-from .shader_factories import make_convolution_shader, make_fft_shader, make_real_rows_shader
+from .shader_factories import make_convolution_shader, make_fft_shader, make_real_rows_shader, register_plan_cache
 
 
 def _torch():
@@ -99,6 +99,9 @@ def plans_for(B, H, W, ctype) -> _Plans:
     if W % 2:
         raise ValueError("fft_convolve2d needs an even last dimension")
     return _Plans(B, H, W, ctype)
+
+
+register_plan_cache(plans_for.cache_clear)
 
 
 def kernel_spectrum(k, plans: _Plans):

@@ -23,7 +23,7 @@ from ..graph import CommandGraph, dispatch
 from ..shader_map import MappingFunction
 from . import precision as prec
 from .shader_factories import (get_transposed_size, make_convolution_shader, make_fft_shader,
-                               make_transpose_shader)
+                               make_transpose_shader, register_plan_cache)
 
 
 def _validate_map_argument_annotations(map_fn: MappingFunction, map_name: str) -> None:
@@ -164,6 +164,7 @@ def _uniform_ragged(n, count, inverse, normalize):
 
 
 _UNIFORM_RAGGED: dict = {}
+register_plan_cache(_UNIFORM_RAGGED.clear)
 
 
 def fft2(buffer: Buffer, graph: CommandGraph = None, print_shader: bool = False, input_map: MappingFunction = None,

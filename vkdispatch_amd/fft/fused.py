@@ -29,7 +29,7 @@ from ..graph import dispatch
 from .hooks import Arg
 from .kernels import CompiledKernel, kernel_symbol, source_symbol
 from .planner import KernelPlan
-from .shader_factories import make_convolution_shader, make_fft_shader
+from .shader_factories import make_convolution_shader, make_fft_shader, register_plan_cache
 
 NUM_XCD = 8
 _FUSED_RADICES = {256: (4, 4, 4, 4), 512: (8, 8, 8), 1024: (16, 16, 4)}
@@ -290,6 +290,9 @@ def fused_plan(batch: int, H: int, W: int, normalize: bool = True, mode: str = "
     if mode == "fused":
         return Conv2DFused(batch, H, W, normalize, **_FUSED_TILING.get(W, {}))
     return Conv2DPhases(batch, H, W, normalize, **_PHASE_TILING.get(W, {}))
+
+
+register_plan_cache(fused_plan.cache_clear)
 
 
 def fused_eligible(buffer, kernel, kernel_map=None, buffer_shape=None, transposed_kernel=False,

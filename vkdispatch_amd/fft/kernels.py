@@ -328,6 +328,9 @@ def precompile(kernels, workers: int = 0) -> int:
         return 0
     if workers <= 0:
         workers = min(16, os.cpu_count() or 4)
+    if rt.device_initialised():
+        # never start worker processes from a process that holds a GPU context
+        workers = 1
     if workers == 1 or len(srcs) == 1:
         for s in srcs:
             compile_source(s)

@@ -29,6 +29,7 @@ from ..graph import CommandGraph, dispatch
 from .hooks import Arg
 from .kernels import CompiledKernel, kernel_symbol, translation_unit
 from .planner import KernelPlan
+from .shader_factories import register_plan_cache
 
 _SCALAR_CPP = {dt.complex_bf16: "vkx::bf16", dt.complex32: "_Float16", dt.complex64: "float"}
 
@@ -188,6 +189,9 @@ class Mixed3D:
 @lru_cache(maxsize=8)
 def _plan(shape, in_type, inverse, normalize) -> Mixed3D:
     return Mixed3D(shape, in_type, inverse, normalize)
+
+
+register_plan_cache(_plan.cache_clear)
 
 
 def fft3_mixed(out: Buffer, inp: Buffer, inverse: bool = False, normalize: bool = True,

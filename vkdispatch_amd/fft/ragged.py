@@ -447,6 +447,7 @@ class RaggedFFT:
         if tile_min is None:
             tile_min = auto_tile_min(self.sizes, 2 * int(dev.compute_units))
         self.tile_min = int(tile_min)
+        self.sync = sync
         q = plan_items(self.sizes, self.inverse, self.normalize, self.max_radix, four_step, tables(), sync,
                        self.tile_min, int(fill))
         self._items, self._src, self._dst = q.items, q.src, q.dst
@@ -497,6 +498,8 @@ class RaggedFFT:
 
         def launch(stream=None):
             _ = keep
+            if self.sync == "counter":
+                rt.note_async_check(self, rt.current_stream() if stream is None else stream)
             for i, (a, b) in enumerate(self._segments):
                 _launch_tasks(self._items[a:b], self.scratch_per_wg, self.grid, self.block_first if i == 0 else self.block,
                               stream,
@@ -521,14 +524,26 @@ class RaggedFFT:
         except Exception:
             pass
 
-    def aborted(self) -> bool:
-        """True when the last launch gave up a dependency wait (its result is invalid).
-        Synchronises the device."""
-        rt.synchronize()
+    def _abort_flag(self) -> bool:
         flag = np.zeros(1, np.int32)
         rt.dtoh(flag.ctypes.data, self._ctr + 4, 4)
-        rt.synchronize()
         return bool(flag[0])
+
+    def aborted(self) -> bool:
+        """True when the last launch gave up a dependency wait (its result is invalid).
+        Synchronises the stream; the flag is then consumed here instead of being raised by
+        the next synchronize()."""
+        s = rt.current_stream()
+        rt.check(rt.lib().vkx_stream_synchronize(s), "vkx_stream_synchronize")
+        rt.forget_async_check(self)
+        return self._abort_flag()
+
+    def async_error(self):
+        """Message for synchronize() when the last counter-mode launch gave up a wait."""
+        if self._abort_flag():
+            return (f"ragged batch of {len(self.sizes)} transforms: a dependency wait gave up after "
+                    f"{self.spin_limit or 1 << 20} polls; the result is invalid")
+        return None
 
     def flops(self) -> float:
         return float(np.sum(5.0 * self.sizes * np.log2(np.maximum(self.sizes, 2))))

@@ -226,9 +226,22 @@ def print_cache_info():
     print(get_convoliution_cache_info())
 
 
+# Caches of plans that own device memory (pair-convolution scratch, mixed-precision
+# intermediates, ragged-batch queues): cache_clear() empties them too, so their
+# buffers are released once no CommandGraph that recorded them is alive.
+_PLAN_CACHE_CLEARERS = []
+
+
+def register_plan_cache(clear) -> None:
+    """Register a callable that empties one module-level cache of plans."""
+    _PLAN_CACHE_CLEARERS.append(clear)
+
+
 def cache_clear():
     make_convolution_shader.cache_clear()
     make_fft_shader.cache_clear()
     make_transpose_shader.cache_clear()
     make_real_rows_shader.cache_clear()
     get_transposed_size.cache_clear()
+    for clear in _PLAN_CACHE_CLEARERS:
+        clear()

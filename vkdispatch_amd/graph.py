@@ -21,7 +21,8 @@ class CommandGraph:
     def __init__(self, reset_on_submit: bool = False, submit_on_record: bool = False):
         self.reset_on_submit = reset_on_submit
         self.submit_on_record = submit_on_record
-        self._records: List[Tuple[object, List[int], bytes]] = []
+        # (kernel or launcher, device pointers, packed constants, objects kept alive)
+        self._records: List[Tuple[object, Optional[List[int]], Optional[bytes], tuple]] = []
         self._exec: Optional[int] = None
         self._exec_stream: Optional[int] = None
 
@@ -31,7 +32,11 @@ class CommandGraph:
         if self.submit_on_record:
             kernel.launch_raw(ptrs, consts)
             return
-        self._records.append((kernel, ptrs, consts))
+        # The record holds the Buffer objects themselves, not only their addresses: a
+        # buffer owned by a cached plan (pair-convolution or mixed-precision scratch)
+        # must outlive the plan's eviction from its cache while this graph can replay.
+        keep = tuple(v for v in values if hasattr(v, "ptr"))
+        self._records.append((kernel, ptrs, consts, keep))
         self._drop_exec()
 
     def record_callable(self, fn) -> None:
@@ -39,7 +44,7 @@ class CommandGraph:
         if self.submit_on_record:
             fn(rt.current_stream())
             return
-        self._records.append((fn, None, None))
+        self._records.append((fn, None, None, ()))
         self._drop_exec()
 
     def __len__(self):
@@ -51,7 +56,7 @@ class CommandGraph:
             self._exec = None
 
     def _replay(self, stream: int) -> None:
-        for k, ptrs, consts in self._records:
+        for k, ptrs, consts, _keep in self._records:
             if ptrs is None:
                 k(stream)
             else:

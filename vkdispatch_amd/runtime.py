@@ -183,6 +183,11 @@ def ensure_device(index: int = 0):
     return _device[1]
 
 
+def device_initialised() -> bool:
+    """True once this process has initialised the HIP device (ensure_device)."""
+    return _device is not None
+
+
 def device_arch() -> str:
     return ensure_device().arch.decode().split(":")[0]
 
@@ -223,9 +228,43 @@ class stream_scope:
         return False
 
 
+# Launches whose kernels wait on other workgroups with a bounded spin (the ragged
+# batch's counter mode, the single-launch 2-D convolution) report a wait that gave
+# up through a device flag instead of hanging.  Such a launch registers its plan
+# here; synchronize() reads the flags of the plans launched on the synchronised
+# stream and raises RuntimeError if one is set.  Graph replays are checked the
+# same way when the plan registered itself at record time.
+_async_lock = threading.Lock()
+_async_pending = {}  # stream -> {id(plan): plan}
+
+
+def note_async_check(plan, stream) -> None:
+    """Check ``plan.async_error()`` (a message or None; called after the stream is
+    synchronised, may read device memory) at the next synchronize() of ``stream``."""
+    with _async_lock:
+        _async_pending.setdefault(int(stream), {})[id(plan)] = plan
+
+
+def forget_async_check(plan) -> None:
+    """Drop ``plan`` from every pending check (its caller has read the flag itself)."""
+    with _async_lock:
+        for pending in _async_pending.values():
+            pending.pop(id(plan), None)
+
+
+def _raise_async_errors(stream: int) -> None:
+    with _async_lock:
+        plans = list(_async_pending.pop(int(stream), {}).values())
+    errors = [m for m in (p.async_error() for p in plans) if m]
+    if errors:
+        raise RuntimeError("; ".join(errors))
+
+
 def synchronize(stream=None) -> None:
     s = current_stream() if stream is None else int(stream)
     check(lib().vkx_stream_synchronize(s), "vkx_stream_synchronize")
+    if _async_pending:
+        _raise_async_errors(s)
 
 
 # ---------------------------------------------------------------------------
