@@ -46,46 +46,74 @@ def conv_inputs(batch=64, n=512, seed=0):
 
 
 def _cpu_info():
-    model = ""
+    """CPU model and core counts of this host (from /proc/cpuinfo and lscpu)."""
+    info = {"os_cpu_count": os.cpu_count()}
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
                 if line.startswith("model name"):
-                    model = line.split(":", 1)[1].strip()
+                    info["model"] = line.split(":", 1)[1].strip()
                     break
     except OSError:
         pass
-    return model, os.cpu_count()
-
-
-def cpu_baseline_conv(x, K, budget_s=12.0):
-    """FP64 oracle (single core) on a bounded sample of images, scaled to GB/s."""
-    import oracle as O
-    n = x.shape[-1]
-    per_img_bytes = 2 * n * n * 4
-    imgs, t0 = 0, time.perf_counter()
-    while time.perf_counter() - t0 < budget_s and imgs < x.shape[0]:
-        O.convolve2DR(x[imgs:imgs + 1], K, kernel_inner_only=True)
-        imgs += 1
-    dt = time.perf_counter() - t0
-    oracle_gbs = imgs * per_img_bytes / dt / 1e9
-    extra = None
     try:
-        import scipy.fft as sf
-        w = os.cpu_count() or 1
-        Kc = K.astype(np.complex64)
-        sf.irfft2(sf.rfft2(x[:4], workers=w) * np.conj(Kc), s=(n, n), workers=w)
-        reps, t0 = 0, time.perf_counter()
-        while time.perf_counter() - t0 < 5.0 and reps < 20:
-            sf.irfft2(sf.rfft2(x, workers=w) * np.conj(Kc), s=(n, n), workers=w)
-            reps += 1
-        dt2 = (time.perf_counter() - t0) / reps
-        extra = {"value": (x.nbytes * 2 + Kc.nbytes) / dt2 / 1e9, "unit": "GB/s", "cores": w,
-                 "kind": "pocketfft (scipy.fft, float32)", "sample": f"{x.shape[0]} images x {reps} reps"}
-    except Exception as e:  # scipy missing: report only the oracle
-        extra = {"error": str(e)}
-    return {"value": oracle_gbs, "unit": "GB/s", "cores": 1, "kind": "port",
-            "sample": f"{imgs} of {x.shape[0]} images through the FP64 oracle in {dt:.1f} s"}, extra
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)",
+                             "NUMA node(s)", "CPU max MHz"):
+                info["lscpu " + k.strip()] = v.strip()
+    except Exception:
+        pass
+    try:
+        info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    except Exception:
+        pass
+    return info
+
+
+def _median_time(fn, warm=3, reps=10, budget_s=20.0):
+    """Median wall time of fn() over up to ``reps`` runs after ``warm`` warm-ups,
+    stopping early once ``budget_s`` is spent (at least 3 timed runs)."""
+    for _ in range(warm):
+        fn()
+    ts, t_start = [], time.perf_counter()
+    while len(ts) < reps and (len(ts) < 3 or time.perf_counter() - t_start < budget_s):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), len(ts)
+
+
+def _workers():
+    try:
+        return len(os.sched_getaffinity(0))
+    except Exception:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline_conv(x, K):
+    """The reference's CPU path cannot run here (its OpenCL backend needs pyopencl and
+    PoCL, absent, and the reference is not on this host); the stated substitute is
+    multithreaded pocketfft (scipy.fft, float32 / complex64 kept in single precision,
+    workers = the cores this process may use), timed as 3 warm-ups + median of 10 on
+    the whole 64-image workload; single-threaded numpy.fft on 8 images beside it."""
+    import scipy.fft as sf
+    n = x.shape[-1]
+    w = _workers()
+    Kc = K.astype(np.complex64)
+    nbytes = x.nbytes * 2 + Kc.nbytes
+    t, reps = _median_time(lambda: sf.irfft2(sf.rfft2(x, workers=w) * np.conj(Kc), s=(n, n), workers=w))
+    xs = x[:8]
+    t1, reps1 = _median_time(lambda: np.fft.irfft2(np.fft.rfft2(xs) * np.conj(Kc), s=(n, n)), warm=1, reps=3)
+    return {"value": round(nbytes / t / 1e9, 3), "unit": "GB/s", "cores": w, "kind": "port",
+            "impl": "scipy.fft (pocketfft C++), float32, irfft2(rfft2(x) * conj(K)); substitute for the "
+                    "reference's OpenCL/PoCL path, which cannot run on this host",
+            "ms": round(t * 1e3, 3),
+            "sample": f"all 64 images per run, median of {reps} runs after 3 warm-ups"}, \
+        {"value": round((xs.nbytes * 2 + Kc.nbytes) / t1 / 1e9, 3), "unit": "GB/s", "cores": 1,
+         "impl": "numpy.fft single thread (float64 internally)", "sample": f"8 images, median of {reps1} runs"}
 
 
 def committed_traffic(kernel_name):
@@ -106,6 +134,7 @@ def committed_traffic(kernel_name):
 def bench_conv(steps, warmup, args):
     import vkdispatch_amd as vd
     from vkdispatch_amd.fft import fused
+    from vkdispatch_amd.fft.kernels import source_symbol
     import oracle as O
 
     batch, n = 64, 512
@@ -121,84 +150,146 @@ def bench_conv(steps, warmup, args):
         bufs.append(b)
     vd.synchronize()
 
-    # each step convolves a fresh copy of the input (no step re-reads another's data);
-    # launched directly (back to back on the stream) and as one replayed hipGraph per
-    # step, the faster of the two is the bench value (both reported)
+    def restore():
+        for b in bufs:
+            vd.runtime.dtod(b.ptr, src.ptr, src.mem_size)
+        vd.synchronize()
+
+    plan = fused.fused_eligible(bufs[0], kbuf, kernel_inner_only=True)
+    kernels = plan.kernels() if plan is not None else list(fused.composed_kernels((batch, n, n)))
+    names = [source_symbol(kk.source) for kk in kernels]
+
+    # 1. the bench value: every step convolves its own fresh copy of the input (all
+    # copies resident in HBM before timing, 25 x 64 MiB: more than the MALL holds),
+    # launched back to back on the stream
     def run_direct(i):
         vd.fft.convolve2DR(bufs[i], kbuf, kernel_inner_only=True)
 
+    for i in range(warmup):
+        run_direct(i)
+    vd.synchronize()
+    e0, e1 = vd.Event(), vd.Event()
+    t0 = time.perf_counter()
+    e0.record()
+    for i in range(warmup, warmup + steps):
+        run_direct(i)
+    e1.record()
+    vd.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = e0.elapsed_ms(e1) / steps
+    ms = wall / steps * 1e3
+    alg_bytes = batch * n * n * 4 * 2 + n * (n // 2 + 1) * 8
+    value = alg_bytes / (wall / steps) / 1e9
+
+    # parity: every image of the last step against the FP64 oracle
+    res = bufs[-1].read_real(0)
+    ref = O.convolve2DR(x, K, kernel_inner_only=True)
+    per_img = [O.rel_l2(res[b], ref[b]) for b in range(batch)]
+
+    # 2. per-kernel times in the same fresh-input loop: events between the launches
+    restore()
+    vals = plan.values(bufs[0], kbuf) if plan is not None else None
+    evs = [[vd.Event() for _ in range(len(kernels) + 1)] for _ in range(steps)]
+    for i in range(warmup):
+        run_direct(i)
+    for j, i in enumerate(range(warmup, warmup + steps)):
+        evs[j][0].record()
+        if plan is not None:
+            vals = plan.values(bufs[i], kbuf)
+            for kk, ev in zip(kernels, evs[j][1:]):
+                kk.launch(vals)
+                ev.record()
+        else:
+            run_direct(i)
+            evs[j][-1].record()
+    vd.synchronize()
+    per_kernel = []
+    if plan is not None:
+        bytes_of = _conv_kernel_bytes(plan, batch, n)
+        for kidx, name in enumerate(names):
+            d = [evs[j][kidx].elapsed_ms(evs[j][kidx + 1]) for j in range(steps)]
+            per_kernel.append({"name": name, "ms_mean": float(np.mean(d)), "ms_median": float(np.median(d)),
+                               "algorithmic_bytes": bytes_of[kidx]})
+    dom = max(per_kernel, key=lambda p: p["ms_mean"]) if per_kernel else None
+
+    # 3. the same steps as one replayed hipGraph each (reported, not the value)
+    restore()
     graphs = []
     for b in bufs:
         g = vd.CommandGraph()
         vd.fft.convolve2DR(b, kbuf, kernel_inner_only=True, graph=g)
-        graphs.append(g)
-    for g in graphs:
         g.capture()
-
-    def timed_wall(run):
-        for i in range(warmup):
-            run(i)
-        vd.synchronize()
-        t0 = time.perf_counter()
-        for i in range(warmup, warmup + steps):
-            run(i)
-        vd.synchronize()
-        return time.perf_counter() - t0
-
-    wall_graph = timed_wall(lambda i: graphs[i].submit())
-    for b in bufs:  # restore the inputs
-        vd.runtime.dtod(b.ptr, src.ptr, src.mem_size)
+        graphs.append(g)
+    for i in range(warmup):
+        graphs[i].submit()
     vd.synchronize()
-    wall_direct = timed_wall(run_direct)
-    wall = min(wall_graph, wall_direct)
-    ms = wall / steps * 1e3
+    t0 = time.perf_counter()
+    for i in range(warmup, warmup + steps):
+        graphs[i].submit()
+    vd.synchronize()
+    wall_graph = time.perf_counter() - t0
 
-    alg_bytes = batch * n * n * 4 * 2 + n * (n // 2 + 1) * 8
-    value = alg_bytes / (wall / steps) / 1e9
-
-    # parity on two images of the last step (same input as every step)
-    res = bufs[-1].read_real(0)
-    ref = O.convolve2DR(x[:2], K, kernel_inner_only=True)
-    rel = O.rel_l2(res[:2], ref)
-
-    # dominant-kernel timing with HIP events on the launch stream
-    prof = fused.profile_convolve2DR(bufs[0], kbuf, reps=20)
-    dom = max(prof, key=lambda p: p["ms"])
-    achieved = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": committed_traffic(dom["name"]),
-                "kernel": dom["name"],
-                "kernel_ms": round(dom["ms"], 5), "kernel_bytes": dom["bytes"]}
+    roofline = None
+    if dom is not None:
+        achieved = dom["algorithmic_bytes"] / (dom["ms_mean"] * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": committed_traffic(dom["name"]),
+                    "kernel": dom["name"], "kernel_ms": round(dom["ms_mean"], 5),
+                    "kernel_algorithmic_bytes": dom["algorithmic_bytes"],
+                    "timing": "HIP events on the launch stream around each launch of the timed fresh-input loop"}
     out = {
         "metric": "FFT-conv GB/s & %HBM roofline; BF16-MFMA vs FP32 speedup at rel-L2<=1e-3",
         "value": round(value, 2), "unit": "GB/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
         "ms_per_step": round(ms, 5), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "fp32 2-D FFT-convolution, 64 x 512x512 real images, one 512x512 kernel "
-                               "(Gaussian sigma=4 + 1% noise) broadcast, convolve2DR in place",
-                   "batch": batch, "shape": [n, n], "algorithmic_bytes_per_step": alg_bytes},
+        "config": {"workload": "fp32 2-D FFT-convolution, 64 x 512x512 real images U(-1,1) seed 0, one 512x512 "
+                               "kernel (Gaussian sigma=4 + 1% noise) as a pre-transformed half spectrum, broadcast; "
+                               "convolve2DR in place, each step on its own HBM-resident copy",
+                   "batch": batch, "shape": [n, n], "algorithmic_bytes_per_step": alg_bytes,
+                   "path": type(plan).__name__ if plan is not None else "three-dispatch composition"},
         "roofline": roofline,
-        "hbm_fraction_of_step": round(value / HBM_PEAK_GBS, 4),
-        "ms_per_step_direct": round(wall_direct / steps * 1e3, 5),
+        "step_frac": round(value / HBM_PEAK_GBS, 4),
+        "gpu_ms_per_step": round(gpu_ms, 5),
         "ms_per_step_graph": round(wall_graph / steps * 1e3, 5),
-        "parity_rel_l2_vs_fp64": rel,
-        "kernels": prof,
+        "parity": {"images_checked": batch, "worst_rel_l2_vs_fp64": max(per_img),
+                   "batch_rel_l2_vs_fp64": O.rel_l2(res, ref), "bound": 1e-5},
+        "kernels": per_kernel,
     }
     if not args.no_cpu:
-        base, extra = cpu_baseline_conv(x, K)
+        base, single = cpu_baseline_conv(x, K)
         out["cpu_baseline"] = base
-        out["cpu_baseline_pocketfft"] = extra
-        model, ncpu = _cpu_info()
-        out["cpu_host"] = {"model": model, "os_cpu_count": ncpu}
+        out["cpu_numpy_single_thread"] = single
+        out["cpu_host"] = _cpu_info()
     return out
 
 
+def _conv_kernel_bytes(plan, batch, n):
+    """Algorithmic bytes of each kernel of a pair-packed plan: the single launch moves
+    the step's bytes; the phase kernels each move their own inputs and outputs."""
+    from vkdispatch_amd.fft import fused
+    wh = n // 2 + 1
+    real_b = batch * n * n * 4
+    kb = n * wh * 8
+    if isinstance(plan, fused.Conv2DXcd):
+        return [2 * real_b + kb]
+    cplx_b = (batch + 1) // 2 * n * n * 8
+    kt_b = n * n * 8
+    return [real_b + cplx_b + kb + kt_b, 2 * cplx_b + kt_b, cplx_b + real_b]
+
+
 def bench_c2c(steps, warmup, args):
+    """C2C N=4096 x 1024 complex64 (U = N(0,1) real and imaginary parts, seed 0):
+    vd.fft.fft then vd.fft.ifft in place per step, replayed as one captured graph."""
     import vkdispatch_amd as vd
     import oracle as O
     rng = np.random.default_rng(0)
     x = (rng.standard_normal((1024, 4096)) + 1j * rng.standard_normal((1024, 4096))).astype(np.complex64)
     buf = vd.asbuffer(x)
+    # parity of one forward transform against numpy.fft in FP64
+    vd.fft.fft(buf)
+    fwd = buf.read(0)
+    rel_fwd = O.rel_l2(fwd, np.fft.fft(x.astype(np.complex128), axis=-1))
+    buf.write(x)
     g = vd.CommandGraph()
     vd.fft.fft(buf, graph=g)
     vd.fft.ifft(buf, graph=g)
@@ -206,19 +297,32 @@ def bench_c2c(steps, warmup, args):
     for _ in range(warmup):
         g.submit()
     vd.synchronize()
+    e0, e1 = vd.Event(), vd.Event()
     t0 = time.perf_counter()
+    e0.record()
     for _ in range(steps):
         g.submit()
+    e1.record()
     vd.synchronize()
     wall = (time.perf_counter() - t0) / steps
     alg = 2 * 2 * x.nbytes
     res = buf.read(0)
     k_fwd = vd.fft.make_fft_shader((1024, 4096), None)
     ms_fwd = _time_kernel(k_fwd, [buf], 20)
-    return {"metric": "C2C N=4096 x 1024 fwd+inv GB/s", "value": round(alg / wall / 1e9, 2), "unit": "GB/s",
-            "ms_per_step": round(wall * 1e3, 5), "steps": steps, "warmup": warmup,
-            "fwd_kernel_ms": ms_fwd, "fwd_kernel_gbs": round(2 * x.nbytes / (ms_fwd * 1e-3) / 1e9, 1),
-            "roundtrip_rel_l2": O.rel_l2(res, x)}
+    out = {"metric": "C2C N=4096 x 1024 fwd+inv GB/s", "value": round(alg / wall / 1e9, 2), "unit": "GB/s",
+           "ms_per_step": round(wall * 1e3, 5), "gpu_ms_per_step": round(e0.elapsed_ms(e1) / steps, 5),
+           "steps": steps, "warmup": warmup,
+           "gflops": round(2 * 5 * 4096 * 12 * 1024 / wall / 1e9, 1),
+           "fwd_kernel_ms": round(ms_fwd, 5), "fwd_kernel_gbs": round(2 * x.nbytes / (ms_fwd * 1e-3) / 1e9, 1),
+           "fwd_kernel_frac": round(2 * x.nbytes / (ms_fwd * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+           "parity": {"fwd_rel_l2_vs_fp64": rel_fwd, "roundtrip_rel_l2": O.rel_l2(res, x), "bound": 5e-6}}
+    if not getattr(args, "no_cpu", False):
+        import scipy.fft as sf
+        w = _workers()
+        t, reps = _median_time(lambda: sf.ifft(sf.fft(x, axis=-1, workers=w), axis=-1, workers=w))
+        out["cpu_pocketfft"] = {"value": round(alg / t / 1e9, 3), "unit": "GB/s", "cores": w,
+                                "ms": round(t * 1e3, 3), "sample": f"fft+ifft of the whole batch, median of {reps}"}
+    return out
 
 
 def bench_ragged(steps, warmup, args):
@@ -256,10 +360,24 @@ def bench_ragged(steps, warmup, args):
         worst = max(worst, O.rel_l2(out[off:off + n], np.fft.fft(x.astype(np.complex64).astype(np.complex128))))
         off += n
     alg = 2 * packed.nbytes
-    return {"metric": "ragged 4096 x C2C (64..65536) GFLOP/s", "value": round(plan.flops() / wall / 1e9, 1),
-            "unit": "GFLOP/s", "ms_per_step": round(wall * 1e3, 4), "gpu_ms_per_step": round(gpu_ms, 4),
-            "GB_s": round(alg / wall / 1e9, 1), "points": int(sum(sizes)), "tuned": tuned,
-            "breakdown": plan.breakdown(), "worst_rel_l2_vs_fp64": worst}
+    res = {"metric": "ragged 4096 x C2C (64..65536) GFLOP/s", "value": round(plan.flops() / wall / 1e9, 1),
+           "unit": "GFLOP/s", "ms_per_step": round(wall * 1e3, 4), "gpu_ms_per_step": round(gpu_ms, 4),
+           "GB_s": round(alg / wall / 1e9, 1), "hbm_frac": round(alg / wall / 1e9 / HBM_PEAK_GBS, 4),
+           "points": int(sum(sizes)), "tuned": {k: tuned[k] for k in ("best", "cached") if k in tuned},
+           "breakdown": plan.breakdown(), "parity": {"worst_rel_l2_vs_fp64": worst, "bound": 1e-5}}
+    if not getattr(args, "no_cpu", False):
+        import scipy.fft as sf
+        w = _workers()
+        xs = [p.astype(np.complex64) for p in parts]
+
+        def cpu():
+            for v in xs:
+                sf.fft(v, workers=w)
+        t, reps = _median_time(cpu, warm=1, reps=5)
+        res["cpu_pocketfft"] = {"value": round(plan.flops() / t / 1e9, 2), "unit": "GFLOP/s", "cores": w,
+                                "ms": round(t * 1e3, 2),
+                                "sample": f"one scipy.fft.fft call per transform, all 4096, median of {reps}"}
+    return res
 
 
 def bench_mixed3d(steps, warmup, args):
@@ -340,9 +458,16 @@ def bench_mixed3d(steps, warmup, args):
                       "mfma_utilisation": plan.mfma_count() * 16384 / (ms * 1e-3) / MFMA_PEAK_FLOPS,
                       "speedup_vs_fp32": ms32 / ms}
     del bufs
-    return {"metric": "3-D C2C 256^3: BF16-MFMA split-precision vs FP32 speedup at rel-L2<=1e-3",
-            "value": round(res["bf16"]["speedup_vs_fp32"], 3), "unit": "x", "steps": steps, "warmup": warmup,
-            "detail": res}
+    out = {"metric": "3-D C2C 256^3: BF16-MFMA split-precision vs FP32 speedup at rel-L2<=1e-3",
+           "value": round(res["bf16"]["speedup_vs_fp32"], 3), "unit": "x", "steps": steps, "warmup": warmup,
+           "detail": res}
+    if not getattr(args, "no_cpu", False):
+        import scipy.fft as sf
+        w = _workers()
+        t, reps = _median_time(lambda: sf.fftn(z, workers=w), warm=1, reps=5)
+        out["cpu_pocketfft"] = {"value": round(t * 1e3, 2), "unit": "ms", "cores": w,
+                                "sample": f"scipy.fft.fftn of the complex64 volume, median of {reps}"}
+    return out
 
 
 def bench_autograd(steps, warmup, args):
@@ -389,12 +514,31 @@ def bench_autograd(steps, warmup, args):
     # FFT path bytes per step: forward reads x and k, writes y; backward reads g and x,
     # writes dx and dk (the loss's own elementwise kernels are not counted)
     alg = (img + H * W * 4 + img) + (2 * img + img + H * W * 4)
-    return {"metric": "differentiable 2-D FFT convolution, forward + backward per step", "value": round(ms, 4),
-            "unit": "ms", "higher_is_better": False, "steps": steps, "warmup": warmup,
-            "config": {"workload": "32 x 1024^2 float32 images, learnable 1024^2 kernel, loss 0.5||y - t||^2"},
-            "fft_path_bytes_per_step": alg, "fft_path_gbs_lower_bound": round(alg / (ms * 1e-3) / 1e9, 1),
-            "torch_fft_autograd_ms": round(ms_vendor, 4), "speedup_vs_torch_fft": round(ms_vendor / ms, 3),
-            "rel_l2_dx_vs_torch_fft": rel(dx, dx_v), "rel_l2_dk_vs_torch_fft": rel(dk, dk_v)}
+    out = {"metric": "differentiable 2-D FFT convolution, forward + backward per step", "value": round(ms, 4),
+           "unit": "ms", "higher_is_better": False, "steps": steps, "warmup": warmup,
+           "config": {"workload": "32 x 1024^2 float32 images, learnable 1024^2 kernel, loss 0.5||y - t||^2"},
+           "fft_path_bytes_per_step": alg, "fft_path_gbs_lower_bound": round(alg / (ms * 1e-3) / 1e9, 1),
+           "torch_fft_autograd_ms": round(ms_vendor, 4), "speedup_vs_torch_fft": round(ms_vendor / ms, 3),
+           "parity": {"rel_l2_dx_vs_torch_fft": rel(dx, dx_v), "rel_l2_dk_vs_torch_fft": rel(dk, dk_v),
+                      "bound": 1e-5}}
+    if not getattr(args, "no_cpu", False):
+        import scipy.fft as sf
+        w = _workers()
+        xc = x.detach().cpu().numpy()
+        kc = k.detach().cpu().numpy()
+        tc = t.cpu().numpy()
+
+        def cpu_step():
+            X = sf.rfft2(xc, workers=w)
+            Kf = sf.rfft2(kc, workers=w)
+            y = sf.irfft2(X * np.conj(Kf), s=(H, W), workers=w)
+            G = sf.rfft2(y - tc, workers=w)
+            sf.irfft2(G * Kf, s=(H, W), workers=w)                              # dx
+            sf.irfft2((X * np.conj(G)).sum(axis=0), s=(H, W), workers=w)        # dk
+        tcpu, reps = _median_time(cpu_step, warm=1, reps=5)
+        out["cpu_pocketfft"] = {"value": round(tcpu * 1e3, 2), "unit": "ms", "cores": w,
+                                "sample": f"forward + adjoint of the whole batch, median of {reps}"}
+    return out
 
 
 def bench_copy(steps, warmup, args):
@@ -435,8 +579,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="conv2d", choices=["conv2d", "c2c4096", "ragged", "mixed3d", "autograd", "copy"])
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--no-mixed", action="store_true", help="conv2d: skip the BF16-MFMA 256^3 comparison")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
+    ap.add_argument("--only", action="store_true",
+                    help="conv2d: the headline workload only (no sub-objects for the other workloads)")
     ap.add_argument("--cached-tune", action="store_true",
                     help="ragged: use the stored autotuner pick for the batch instead of timing the candidates")
     args = ap.parse_args()
@@ -446,17 +591,31 @@ def main():
         sys.exit(2)
     if args.workload == "conv2d":
         out = bench_conv(args.steps, args.warmup, args)
-        if not args.no_mixed:
+        if not args.only:
+            def note(msg):
+                print(msg, file=sys.stderr, flush=True)
             # the metric's second half: BF16-MFMA 256^3 transform vs the FP32 path
+            note("bench: 3-D 256^3 bf16 MFMA vs fp32")
             m = bench_mixed3d(args.steps, args.warmup, args)
             d = m["detail"]
             out["bf16_mfma_vs_fp32"] = {
                 "value": m["value"], "unit": "x", "workload": "3-D C2C 256^3, bf16 storage, fp32 output",
                 "ms_bf16": d["bf16"]["ms"], "ms_fp32": d["fp32"]["ms"],
                 "rel_l2_vs_fp64_quantized": d["bf16"]["rel_l2_vs_fp64_quantized"],
+                "rel_l2_fp32_path_vs_fp64": d["fp32"]["rel_l2_vs_fp64"],
                 "mfma_utilisation": d["bf16"]["mfma_utilisation"],
                 "fp16_storage_speedup": d["fp16"]["speedup_vs_fp32"],
+                "fp16_rel_l2_vs_fp64_quantized": d["fp16"]["rel_l2_vs_fp64_quantized"],
                 "timing": "median of 7 interleaved rounds of back-to-back launches per leg"}
+            if "cpu_pocketfft" in m:
+                out["bf16_mfma_vs_fp32"]["cpu_pocketfft"] = m["cpu_pocketfft"]
+            # the other workloads of the path, each with its own parity and CPU baseline
+            args.cached_tune = True
+            for key, fn, label in (("c2c_4096x1024", bench_c2c, "C2C 4096 x 1024"),
+                                   ("autograd_32x1024sq", bench_autograd, "differentiable convolution"),
+                                   ("ragged_4096", bench_ragged, "ragged batch")):
+                note(f"bench: {label}")
+                out[key] = fn(args.steps, args.warmup, args)
     elif args.workload == "ragged":
         out = bench_ragged(args.steps, args.warmup, args)
     elif args.workload == "mixed3d":

@@ -177,6 +177,12 @@ int vkx_bluestein_b(uint64_t dst, int n, int m, uint64_t stream);
  * (vkdispatch/execution_pipeline/command_graph.py:193-316) and its CUDA-graph
  * capture hook (vkdispatch/execution_pipeline/cuda_graph_capture.py:9-52). */
 int vkx_graph_begin(uint64_t stream);
+/* *capturing = 1 while `stream` is being captured into a graph (by vkx_graph_begin or by
+ * a caller such as torch.cuda.graph), 2 if that capture was invalidated, 0 otherwise.
+ * The capture-state query behind vkdispatch's cuda_graph_capture scope
+ * (vkdispatch/execution_pipeline/cuda_graph_capture.py:22-52): plans refuse to allocate
+ * or initialise device memory while their stream is capturing. */
+int vkx_stream_is_capturing(uint64_t stream, int* capturing);
 int vkx_graph_end(uint64_t stream, uint64_t* graph_exec);
 int vkx_graph_launch(uint64_t graph_exec, uint64_t stream);
 int vkx_graph_destroy(uint64_t graph_exec);

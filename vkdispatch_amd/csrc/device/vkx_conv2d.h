@@ -12,32 +12,19 @@
 // a half spectrum that is not Hermitian on those two columns (the c2r row
 // transform keeps only the real part there).
 //
-// Data flow per image pair, through a complex scratch image S stored tile-major:
-// 16-column tiles, each H rows x 16 columns (128-byte rows) contiguous, element
-// (ky, kx) at (kx / 16) * 16H + ky * 16 + kx % 16.  Row work then reads and writes
-// whole 128-byte lines from registers, and a column tile is one contiguous block.
-//   K  (kernel prep): Kt[kx * H + ky] = conj(Kf[ky][kx]) * SCALE (column-major)
-//   A  (rows [y0, y0+RA)): row FFTs of z, written to S
-//   B  (one tile): staged in LDS column-major; each wave: column FFT, multiply by
-//      its Kt column, column IFFT; tile written back
-//   C  (rows [y0, y0+RA)): row IFFTs from S, real part -> image a, imaginary part
-//      -> image b
-//
-// The phases run either as ordinary kernels in stream order (conv2d_phase_body)
-// or as one cooperative launch (conv2d_pair_body).  In the cooperative form the
-// L2 is per XCD and not coherent across XCDs, so all tasks of one pair run on
-// workgroups of one XCD: there is one task queue per XCD (with its own Kt copy)
-// and a workgroup takes tasks only from the queue of the XCD it runs on (XCC_ID
-// register).  Producers complete their stores (s_waitcnt) before bumping a
-// counter with a device-scope atomic, and consumers read with L1-bypassing
-// (device-scope) loads after seeing the counter; the data stays in the XCD's L2
-// unless evicted.  No agent-scope fence is used: it would write back /
-// invalidate the whole L2.  Workgroups take tickets from their queue's counter
-// (the next ticket is requested while the current task runs); a task only waits
-// for tasks with smaller tickets of the same queue, held by running workgroups,
-// so the grid cannot deadlock; every wait is bounded and a timeout sets an error
-// bit.  The last workgroup to exit runs the tasks of any queue that no workgroup
-// took (an XCD that received no workgroup), then resets the control block.
+// Data flow per image pair: row FFTs of z, then column FFT . Kt . column IFFT,
+// then row IFFTs (real part -> image a, imaginary part -> image b), with
+//   Kt(ky, kx) = conj(Kf[ky][kx]) * SCALE.
+// Two forms:
+//  * conv2d_phase_body: three ordinary kernels in stream order over all pairs,
+//    through a column-major scratch image per pair (S[kx * H + ky]) and a
+//    column-major Kt prepared by the first kernel.
+//  * conv2d_xcd_body: one launch; the three phases become tasks of per-XCD queues
+//    and the transposes between them stay inside one XCD's L2 (description at the
+//    body).  Its scratch is tile-major: 16-column tiles, each H rows x 16 columns
+//    (128-byte rows) contiguous, element (ky, kx) at (kx / 16) * 16H + ky * 16 +
+//    kx % 16, so row tasks read and write whole 128-byte lines from registers and a
+//    column tile is one contiguous block.
 #pragma once
 #include "vkx_fft.h"
 
@@ -52,19 +39,10 @@ template <int N_, int NP_> struct WaveCfg {
 
 // Register-resident twiddles (loaded once per task, see TwSet) when a lane's set is
 // small: 14 values for 512 = 8.8.8 and 9 for 256 = 4.4.4.4.  The 27 of 1024 =
-// 16.16.4 would push the cooperative 1024 kernel into spills, so those kernels keep
-// reading the table.
+// 16.16.4 keep reading the table (registers are better spent on data there).
 template <class WC, class L> constexpr bool conv_twreg() { return list_tw_regs<WC, L>::value <= 16; }
 
-// control block layout (u32 words)
-constexpr u32 CTL_TICKET = 0;      // + 32 q
-constexpr u32 CTL_EXIT = 256;      // workgroups that have left
-constexpr u32 CTL_ERROR = 512;     // bit 0: a wait timed out; bit 2: the last workgroup served a queue
-constexpr u32 CTL_KDONE = 768;     // + 32 q: kernel-prep tasks done for queue q
-constexpr u32 CTL_PAIRS = 1024;    // + 64 p: A-done at +0, B-done at +32
-constexpr int CONV_QUEUES = 8;
 constexpr u32 CONV_SPIN_LIMIT = 1u << 18;
-constexpr u32 PHASE_A = 0, PHASE_B = 1, PHASE_C = 2, PHASE_K = 3;
 
 __device__ __forceinline__ u32 ld_agent(const u32* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -85,8 +63,8 @@ __device__ __forceinline__ cpx<float> ld_c64_l2(const cpx<float>* p) {
   return r;
 }
 
-// scratch / Kt load: L1-bypassing inside the cooperative kernel (data written by
-// other workgroups of this launch), an ordinary cached load in the phase kernels
+// scratch / Kt load: L1-bypassing where other workgroups of the same launch wrote
+// the data, an ordinary cached load in the phase kernels
 template <bool L2ONLY> __device__ __forceinline__ cpx<float> ld_c64(const cpx<float>* p) {
   if constexpr (L2ONLY) return ld_c64_l2(p);
   else return *p;
@@ -480,91 +458,353 @@ __device__ __forceinline__ void conv2d_phase_body(const KParams& prm) {
   }
 }
 
-// The phases in one cooperative launch (see the top of this file).  Slot 3: the
-// control block; slot 4: the task table; slot 5: one Kt per queue.
+// ---------------------------------------------------------------------------
+// Single-launch form, XCD-resident exchange (conv2d_xcd_body).
+//
+// The three phases per image pair as tasks of one launch.  Each XCD has its own
+// task queue and its own NSLOT pair spectra of scratch (the slot images), and a
+// workgroup takes tasks only from the queue of the XCD it runs on (XCC_ID
+// register), so every value a task reads from scratch was written on the same XCD
+// and is found in that XCD's L2 (the transposes never leave the XCD unless the L2
+// evicts them).  Pair i of queue q is pair q + 8 i and lives in slot i % NSLOT.
+//
+// Tasks (slice s = RA rows, tile c = 16 columns):
+//   A(i, s)     row FFTs of pair i, rows of slice s -> slot (tile-major layout)
+//   B(i, c)     column FFT . Kt . column IFFT of tile c, in place in the slot;
+//               Kt is built from the half spectrum K in LDS by the task itself
+//   CA(i, s)    row IFFTs of pair i, slice s -> images, then A(i + NSLOT, s):
+//               the next pair of the slot enters the rows pair i just left
+//   C(i, s)     row IFFTs only (the last NSLOT pairs of a queue)
+// A queue lists A(0..NSLOT-1), then per group of NSLOT pairs all their B tasks,
+// then all their CA / C tasks.  B(i) waits for every row slice of pair i (the A or
+// CA tasks that wrote it); CA(i) / C(i) wait for every tile of B(i).  A task waits
+// only for tasks earlier in its queue, which running workgroups already hold, so
+// the launch cannot deadlock; every wait is bounded (an expired wait sets error
+// bit 0 and the task goes on; the host reports it).
+//
+// Memory ordering inside one XCD: a task's stores complete to L2 (s_waitcnt) before
+// its workgroup bumps the done counter; a waiting workgroup polls the counter, then
+// reads slot data with device-scope loads, which miss the CU's L1 and read the
+// XCD's L2 (or memory behind it).  No agent-scope release is used: on this part it
+// writes back the whole L2, and the data never crosses XCDs.
+//
+// The last workgroup to leave runs, alone and in order, the tasks of any queue no
+// workgroup took (an XCD that received none), then resets the control block, so
+// every launch starts from zero without a memset (graph capture safe).
+// ---------------------------------------------------------------------------
+constexpr u32 XT_A = 0, XT_B = 1, XT_CA = 2, XT_C = 3;
+constexpr u32 XCTL_TICKET = 0;  // + 32 q: next ticket of queue q
+constexpr u32 XCTL_EXIT = 256;  // workgroups that have left
+constexpr u32 XCTL_ERROR = 288;  // bit 0: a wait expired; bit 2: the last workgroup served a queue
+constexpr u32 XCTL_CNT = 512;   // + 64 (q QMAX + i): rows of pair i done at +0, tiles done at +32
+constexpr u32 XCTL_WORDS_FIXED = 512;
+
+template <class C> struct Xcd {
+  using WC = WaveCfg<C::W, C::NP>;
+  static constexpr int NV = C::W / 64;     // points per lane of one transform
+  static constexpr int RPW = C::RA / C::NW;  // rows per wave in a row task
+  static constexpr int NT = C::NW * 64;
+  static constexpr int CB = 16 / C::NW;    // columns per wave in a tile task
+  static constexpr u32 NS = C::H / C::RA;  // row slices per pair
+  static constexpr u32 NB = C::W / 16;     // column tiles per pair
+  static constexpr u64 PAIR = (u64)C::H * (u64)C::W;
+  static constexpr int IT = 16 * C::H / NT;  // tile elements per thread
+  static constexpr int LDS = 16 * C::NP;     // one tile, column-major, padded
+  static_assert(C::H == C::W && C::W % 64 == 0, "square images, rows a multiple of 64");
+  static_assert(C::RA % C::NW == 0 && 16 % C::NW == 0 && (16 * C::H) % NT == 0, "task split");
+  static_assert(C::NW * C::NP <= LDS, "row exchange regions fit the tile area");
+  static_assert(C::NP >= C::H + C::H / 32, "LDS column region");
+};
+
+template <class C> __device__ __forceinline__ u32* xcd_cnt(u32* ctl, u32 q, u32 i) {
+  return ctl + XCTL_CNT + 64u * (q * (u32)C::QMAX + i);
+}
+
+// the RPW rows of this wave (slice rows y0 + wave + NW j), in the first-pass layout
 template <class C>
-__device__ __forceinline__ void conv2d_pair_body(const KParams& prm) {
-  using SH = ConvShape<C>;
-  static_assert(C::LAYOUT == 0, "the cooperative kernel uses the tile-major layout");
-  __shared__ cpx<float> lds[SH::LDS_ALL];
-  __shared__ u32 s_task;
-  const ConvArgs<C> a = conv_args<C>(prm);
+__device__ __forceinline__ void xcd_load_rows(const float* x, u32 pair, u32 slice, int wave, int lane,
+                                              float (&ra)[Xcd<C>::RPW][Xcd<C>::NV],
+                                              float (&rb)[Xcd<C>::RPW][Xcd<C>::NV]) {
+  using X = Xcd<C>;
+  using F0 = typename pl_first<typename C::fwd>::type;
+  const bool has_b = 2 * pair + 1 < C::NIMG;
+#pragma unroll
+  for (int j = 0; j < X::RPW; ++j) {
+    const u32 y = slice * C::RA + (u32)(wave + C::NW * j);
+    const float* pa = x + (u64)(2 * pair) * C::IMG + y * C::PITCH;
+    const float* pb = pa + C::IMG;
+    for_layout<typename X::WC, F0::R>(lane, [&](auto slot, u32 n) {
+      ra[j][decltype(slot)::value] = pa[n];
+      rb[j][decltype(slot)::value] = has_b ? pb[n] : 0.f;
+    });
+  }
+}
+
+// row FFTs of z = x_a + i x_b into the slot (tile-major: four 128-byte lines per store)
+template <class C, class TS>
+__device__ __forceinline__ void xcd_rows_fwd(const TS& ts, cpx<float>* S, u32 slice, cpx<float>* xch, int wave,
+                                             int lane, const float (&ra)[Xcd<C>::RPW][Xcd<C>::NV],
+                                             const float (&rb)[Xcd<C>::RPW][Xcd<C>::NV]) {
+  using X = Xcd<C>;
+  using FL = typename pl_last<typename C::fwd>::type;
+#pragma unroll
+  for (int j = 0; j < X::RPW; ++j) {
+    const u32 y = slice * C::RA + (u32)(wave + C::NW * j);
+    cpx<float> v[X::NV];
+#pragma unroll
+    for (int k = 0; k < X::NV; ++k) v[k] = cpx<float>{ra[j][k], rb[j][k]};
+    ts.template run<-1>(v, xch, lane, 0);
+    for_layout<typename X::WC, FL::R>(lane, [&](auto slot, u32 kx) { S[tile_index<C>(y, kx)] = v[decltype(slot)::value]; });
+  }
+}
+
+// row IFFTs of the slot rows of this wave; real parts -> image a, imaginary -> image b
+template <class C, class TS>
+__device__ __forceinline__ void xcd_rows_inv(const TS& ts, const cpx<float>* S, float* x, u32 pair, u32 slice,
+                                             cpx<float>* xch, int wave, int lane) {
+  using X = Xcd<C>;
+  using I0 = typename pl_first<typename C::inv>::type;
+  using IL = typename pl_last<typename C::inv>::type;
+  const bool has_b = 2 * pair + 1 < C::NIMG;
+  cpx<float> v[X::RPW][X::NV];
+#pragma unroll
+  for (int j = 0; j < X::RPW; ++j) {  // every load of the task in flight first
+    const u32 y = slice * C::RA + (u32)(wave + C::NW * j);
+    for_layout<typename X::WC, I0::R>(lane, [&](auto slot, u32 kx) {
+      v[j][decltype(slot)::value] = ld_c64_l2(S + tile_index<C>(y, kx));
+    });
+  }
+#pragma unroll
+  for (int j = 0; j < X::RPW; ++j) {
+    const u32 y = slice * C::RA + (u32)(wave + C::NW * j);
+    ts.template run<1>(v[j], xch, lane, 0);
+    float* pa = x + (u64)(2 * pair) * C::IMG + y * C::PITCH;
+    float* pb = pa + C::IMG;
+    for_layout<typename X::WC, IL::R>(lane, [&](auto slot, u32 n) {
+      const cpx<float> z = v[j][decltype(slot)::value];
+      pa[n] = z.x;
+      if (has_b) pb[n] = z.y;
+    });
+  }
+}
+
+// Kt(ky, kx) = conj(Kf[ky][kx]) * SCALE with Kf the full spectrum rebuilt from the
+// half spectrum K (rules at the top of this file).  EDGE: the tile holds column 0 or
+// W/2, whose value averages two entries of K.
+template <class C, bool EDGE> __device__ __forceinline__ cpx<float> xcd_kt(const cpx<float>* K, u32 ky, u32 kx) {
+  const u32 kym = (C::H - ky) & (C::H - 1);
+  const bool lo = kx <= (u32)(C::W / 2);
+  cpx<float> kf = K[lo ? ky * C::KW + kx : kym * C::KW + (C::W - kx)];
+  if (!lo) kf.y = -kf.y;
+  if constexpr (EDGE) {
+    if (kx == 0 || kx == (u32)(C::W / 2)) {
+      const cpx<float> k1 = K[kym * C::KW + kx];
+      kf = cpx<float>{0.5f * (kf.x + k1.x), 0.5f * (kf.y - k1.y)};
+    }
+  }
+  const float sc = (float)C::SCALE;
+  return cpx<float>{kf.x * sc, -kf.y * sc};
+}
+
+// this tile's Kt columns, column-major into LDS (K read in 16-element row runs), four
+// elements per thread in flight at a time
+template <class C, bool EDGE>
+__device__ __forceinline__ void xcd_kt_tile(const cpx<float>* K, u32 tile, cpx<float>* lds, int tid) {
+  using X = Xcd<C>;
+  static_assert(X::IT % 4 == 0, "Kt staging in groups of four");
+#pragma nounroll
+  for (int i0 = 0; i0 < X::IT; i0 += 4) {
+    cpx<float> k[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int e = tid + (i0 + i) * X::NT;
+      k[i] = xcd_kt<C, EDGE>(K, (u32)(e >> 4), tile * 16u + (u32)(e & 15));
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int e = tid + (i0 + i) * X::NT, y = e >> 4, c = e & 15;
+      lds[c * C::NP + y + (y >> 5)] = k[i];
+    }
+  }
+}
+
+// tile task: 16 columns of the slot, in place
+template <class C, class TF, class TI>
+__device__ __forceinline__ void xcd_tile(const TF& tsf, const TI& tsi, const cpx<float>* K, cpx<float>* S, u32 tile,
+                                         cpx<float>* lds, int tid, int wave, int lane) {
+  using X = Xcd<C>;
+  using F0 = typename pl_first<typename C::fwd>::type;
+  using FL = typename pl_last<typename C::fwd>::type;
+  using IL = typename pl_last<typename C::inv>::type;
+  // 1. this tile's Kt columns, built through LDS
+  if (tile == 0 || tile == X::NB / 2) xcd_kt_tile<C, true>(K, tile, lds, tid);
+  else xcd_kt_tile<C, false>(K, tile, lds, tid);
+  __syncthreads();
+  cpx<float> kv[X::CB][X::NV];
+#pragma unroll
+  for (int cc = 0; cc < X::CB; ++cc) {
+    const cpx<float>* col = lds + (wave * X::CB + cc) * C::NP;
+    for_layout<typename X::WC, FL::R>(lane, [&](auto slot, u32 ky) { kv[cc][decltype(slot)::value] = col[ky + (ky >> 5)]; });
+  }
+  __syncthreads();
+  // 2. the tile (16 x H contiguous elements) into LDS, column-major
+  cpx<float>* T = S + tile * (u32)(16 * C::H);
+  {
+    cpx<float> t[X::IT];
+#pragma unroll
+    for (int i = 0; i < X::IT; ++i) t[i] = ld_c64_l2(T + tid + i * X::NT);
+#pragma unroll
+    for (int i = 0; i < X::IT; ++i) {
+      const int e = tid + i * X::NT, y = e >> 4, c = e & 15;
+      lds[c * C::NP + y + (y >> 5)] = t[i];
+    }
+  }
+  __syncthreads();
+  // 3. per wave: its columns, FFT, product, IFFT, in their own LDS regions
+#pragma unroll
+  for (int cc = 0; cc < X::CB; ++cc) {
+    cpx<float>* col = lds + (wave * X::CB + cc) * C::NP;
+    cpx<float> v[X::NV];
+    for_layout<typename X::WC, F0::R>(lane, [&](auto slot, u32 n) { v[decltype(slot)::value] = col[n + (n >> 5)]; });
+    tsf.template run<-1>(v, col, lane, 0);
+#pragma unroll
+    for (int k = 0; k < X::NV; ++k) v[k] = cmul(v[k], kv[cc][k]);
+    block_sync<typename X::WC>();
+    tsi.template run<1>(v, col, lane, 0);
+    block_sync<typename X::WC>();
+    for_layout<typename X::WC, IL::R>(lane, [&](auto slot, u32 n) { col[n + (n >> 5)] = v[decltype(slot)::value]; });
+  }
+  __syncthreads();
+  // 4. the tile back
+#pragma unroll
+  for (int i = 0; i < X::IT; ++i) {
+    const int e = tid + i * X::NT, y = e >> 4, c = e & 15;
+    T[e] = lds[c * C::NP + y + (y >> 5)];
+  }
+}
+
+// Slot 0: the images (in place), 1: the half spectrum K, 2: slot scratch (8 NSLOT
+// pair images), 3: the control block, 4: the task table (9 queue offsets, then task
+// words kind << 30 | local pair << 12 | slice).
+template <class C>
+__device__ __forceinline__ void conv2d_xcd_body(const KParams& prm) {
+  using X = Xcd<C>;
+  using L = typename C::fwd;
+  using LI = typename C::inv;
+  __shared__ cpx<float> lds[X::LDS];
+  __shared__ u32 s_word;
+  float* x = bufp<float>(prm, 0);
+  const cpx<float>* K = bufp<const cpx<float>>(prm, 1);
+  cpx<float>* scr = bufp<cpx<float>>(prm, 2);
   u32* ctl = bufp<u32>(prm, 3);
   const u32* table = bufp<const u32>(prm, 4);
+  const cpx<float>* twf = bufp<const cpx<float>>(prm, SLOT_TW_FWD);
+  const cpx<float>* twi = bufp<const cpx<float>>(prm, SLOT_TW_INV);
 
-  const int tid = (int)threadIdx.x;
-  const int lane = tid & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  // queue = the XCD this workgroup actually runs on
-  const u32 q = __builtin_amdgcn_readfirstlane(__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 7u);
-  u32* err = ctl + CTL_ERROR;
+  const int tid0 = (int)threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(tid0 >> 6);
+  const u32 xq = __builtin_amdgcn_readfirstlane(__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 7u);
+  u32* err = ctl + XCTL_ERROR;
 
-  // qq: the queue the task belongs to (its Kt copy and kernel-prep counter)
-  auto run_task = [&](u32 word, u32 qq) {
-    const u32 phase = word >> 30;
-    const u32 pair = (word >> 12) & 0x3FFFFu;
-    const u32 slice = word & 0xFFFu;
-    u32* cnt = ctl + CTL_PAIRS + 64 * pair;
-    cpx<float>* ktq = a.kt + (u64)qq * (u64)(C::H * C::W);
-    if (phase == PHASE_A) {
-      conv_rows_fwd<C>(a, lds, pair, slice, wave, lane);
+  // Each task reads its lane's twiddles once, in flight with its first data loads
+  // (held across the whole launch they would cost 28 registers in every task).
+  constexpr bool SAME = __is_same(L, LI);
+  using TSF = TwSet<typename X::WC, L, conv_twreg<typename X::WC, L>()>;
+  using TSI = TwSet<typename X::WC, LI, conv_twreg<typename X::WC, LI>()>;
+
+  auto run_task = [&](u32 word, u32 q) {
+    // The thread index passes through an empty asm statement per task, so the
+    // compiler derives every address from it inside the task: hoisted out of the
+    // task loop, the per-thread LDS and global offsets of all task kinds would stay
+    // live for the whole launch and crowd out the data registers.
+    int tid = tid0;
+    asm volatile("" : "+v"(tid));
+    const int lane = tid & 63;
+    cpx<float>* xch = lds + wave * C::NP;
+    const u32 kind = word >> 30;
+    const u32 i = (word >> 12) & 0x3FFFFu;
+    const u32 s = word & 0xFFFu;
+    const u32 pair = q + 8u * i;
+    cpx<float>* S = scr + (u64)(q * (u32)C::NSLOT + i % (u32)C::NSLOT) * X::PAIR;
+    u32* cnt = xcd_cnt<C>(ctl, q, i);
+    if (kind == XT_A) {
+      TSF tsf;
+      tsf.load(lane, twf);
+      float ra[X::RPW][X::NV], rb[X::RPW][X::NV];
+      xcd_load_rows<C>(x, pair, s, wave, lane, ra, rb);
+      xcd_rows_fwd<C>(tsf, S, s, xch, wave, lane, ra, rb);
       signal_done(cnt);
-    } else if (phase == PHASE_B) {
-      wait_count(ctl + CTL_KDONE + 32 * qq, SH::NK, err);
-      wait_count(cnt, SH::NA, err);
-      conv_cols<C, true>(a, ktq, lds, pair, slice, tid, wave, lane);
+    } else if (kind == XT_B) {
+      wait_count(cnt, X::NS, err);
+      TSF tsf;
+      tsf.load(lane, twf);
+      if constexpr (SAME) {
+        xcd_tile<C>(tsf, tsf, K, S, s, lds, tid, wave, lane);
+      } else {
+        TSI tsi;
+        tsi.load(lane, twi);
+        xcd_tile<C>(tsf, tsi, K, S, s, lds, tid, wave, lane);
+      }
       signal_done(cnt + 32);
-    } else if (phase == PHASE_C) {
-      wait_count(cnt + 32, SH::NB, err);
-      conv_rows_inv<C, true>(a, lds, pair, slice, wave, lane);
-      __syncthreads();  // LDS reuse by the next task
     } else {
-      conv_kprep<C>(a.K, ktq, slice, tid);
-      signal_done(ctl + CTL_KDONE + 32 * qq);
+      wait_count(cnt + 32, X::NB, err);
+      TSI tsi;
+      tsi.load(lane, twi);
+      xcd_rows_inv<C>(tsi, S, x, pair, s, xch, wave, lane);
+      if (kind == XT_CA) {
+        // the next pair of this slot enters exactly the rows this wave just read
+        TSF tsf;
+        tsf.load(lane, twf);
+        float ra[X::RPW][X::NV], rb[X::RPW][X::NV];
+        xcd_load_rows<C>(x, pair + 8u * (u32)C::NSLOT, s, wave, lane, ra, rb);
+        xcd_rows_fwd<C>(tsf, S, s, xch, wave, lane, ra, rb);
+        signal_done(xcd_cnt<C>(ctl, q, i + (u32)C::NSLOT));
+      } else {
+        __syncthreads();  // LDS reuse by the next task
+      }
     }
   };
 
-  // ---- take tickets from this XCD's queue ----
+  // ---- take tickets from this XCD's queue (the next one is requested while the
+  // current task runs) ----
   {
-    const u32 t_begin = table[q], ntask = table[q + 1] - table[q];
-    u32* ticket = ctl + CTL_TICKET + 32 * q;
-    if (tid == 0) s_task = add_agent(ticket, 1u);
+    const u32 t_begin = table[xq], ntask = table[xq + 1] - table[xq];
+    u32* ticket = ctl + XCTL_TICKET + 32 * xq;
+    if (tid0 == 0) s_word = add_agent(ticket, 1u);
     __syncthreads();
-    u32 cur = __builtin_amdgcn_readfirstlane(s_task);
+    u32 cur = __builtin_amdgcn_readfirstlane(s_word);
     __syncthreads();
     while (cur < ntask) {
       u32 nxt = 0;
-      if (tid == 0) nxt = add_agent(ticket, 1u);  // next ticket, needed after this task
-      run_task(table[9 + t_begin + cur], q);
-      if (tid == 0) s_task = nxt;
+      if (tid0 == 0) nxt = add_agent(ticket, 1u);
+      run_task(table[9 + t_begin + cur], xq);
+      if (tid0 == 0) s_word = nxt;
       __syncthreads();
-      cur = __builtin_amdgcn_readfirstlane(s_task);
+      cur = __builtin_amdgcn_readfirstlane(s_word);
       __syncthreads();
     }
   }
 
-  // ---- the last workgroup to leave serves queues that no workgroup ran on (an
-  // XCD that received none), then resets the control block for the next launch ----
-  if (tid == 0) s_task = add_agent(ctl + CTL_EXIT, 1u);
+  // ---- the last workgroup to leave serves queues no workgroup ran on, then resets
+  // the control block for the next launch ----
+  if (tid0 == 0) s_word = add_agent(ctl + XCTL_EXIT, 1u);
   __syncthreads();
-  const bool last = __builtin_amdgcn_readfirstlane(s_task) == gridDim.x - 1;
+  const bool last = __builtin_amdgcn_readfirstlane(s_word) == gridDim.x - 1;
   __syncthreads();
   if (last) {
-    for (u32 qq = 0; qq < (u32)CONV_QUEUES; ++qq) {
-      if (table[qq + 1] == table[qq] || ld_agent(ctl + CTL_TICKET + 32 * qq) != 0) continue;
-      if (tid == 0) __hip_atomic_fetch_or(err, 4u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (u32 qq = 0; qq < 8u; ++qq) {
+      if (table[qq + 1] == table[qq] || ld_agent(ctl + XCTL_TICKET + 32 * qq) != 0) continue;
+      if (tid0 == 0) __hip_atomic_fetch_or(err, 4u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       for (u32 t = table[qq]; t < table[qq + 1]; ++t) run_task(table[9 + t], qq);
     }
-    if (tid == 0) {
-      for (u32 p = 0; p < C::NPAIRS; ++p) {
-        st_agent(ctl + CTL_PAIRS + 64 * p, 0u);
-        st_agent(ctl + CTL_PAIRS + 64 * p + 32, 0u);
+    __syncthreads();
+    if (tid0 == 0) {
+      for (u32 w = 0; w < 8u * (u32)C::QMAX; ++w) {
+        st_agent(ctl + XCTL_CNT + 64 * w, 0u);
+        st_agent(ctl + XCTL_CNT + 64 * w + 32, 0u);
       }
-      for (u32 qq = 0; qq < (u32)CONV_QUEUES; ++qq) {
-        st_agent(ctl + CTL_TICKET + 32 * qq, 0u);
-        st_agent(ctl + CTL_KDONE + 32 * qq, 0u);
-      }
-      st_agent(ctl + CTL_EXIT, 0u);
+      for (u32 qq = 0; qq < 8u; ++qq) st_agent(ctl + XCTL_TICKET + 32 * qq, 0u);
+      st_agent(ctl + XCTL_EXIT, 0u);
     }
   }
 }

@@ -545,6 +545,14 @@ int vkx_plan_exec(uint64_t plan, const uint64_t* bufs, int num_bufs, const void*
   return 0;
 }
 
+int vkx_stream_is_capturing(uint64_t stream, int* capturing) {
+  *capturing = 0;
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  VKX_HIP(hipStreamIsCapturing(S(stream), &st));
+  *capturing = st == hipStreamCaptureStatusActive ? 1 : (st == hipStreamCaptureStatusInvalidated ? 2 : 0);
+  return 0;
+}
+
 int vkx_graph_begin(uint64_t stream) {
   VKX_HIP(hipStreamBeginCapture(S(stream), hipStreamCaptureModeThreadLocal));
   return 0;

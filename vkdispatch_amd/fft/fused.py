@@ -1,6 +1,6 @@
 """2-D real FFT-convolution (convolve2DR) execution paths and their profiling.
 
-``try_convolve2DR`` runs the single-pass kernel (``csrc/device/vkx_conv2d.h``)
+``try_convolve2DR`` runs the pair-packed kernels (``csrc/device/vkx_conv2d.h``)
 when the call is eligible: a batch of square real images (256, 512 or 1024 on a
 side) in complex64 RFFTBuffers, one kernel spectrum broadcast over the batch,
 the default kernel map.  It returns False otherwise, and the caller falls back
@@ -8,9 +8,10 @@ to the reference's three-dispatch composition (rfft -> convolve(axis ndim-2) ->
 irfft, vkdispatch/fft/functions.py:614-646).  Both paths compute
 irfft2(rfft2(x) . conj(K)) per image (vkdispatch/fft/shader_factories.py:136-145).
 
-The single-pass kernel keeps every image pair on one XCD (one task queue per
-XCD, chosen by the XCC_ID register of the workgroup).  Set VKX_FUSED_CONV=0 to
-force the three-dispatch path.
+Two pair-packed forms: ``Conv2DXcd`` (one launch; per-XCD task queues keep each
+pair's transposes inside one XCD's L2) and ``Conv2DPhases`` (three kernels in
+stream order through a scratch image per pair).  VKX_CONV2D_MODE=xcd|phases
+overrides the choice; VKX_FUSED_CONV=0 forces the three-dispatch composition.
 """
 from __future__ import annotations
 
@@ -27,14 +28,15 @@ from .. import runtime as rt
 from ..buffer import Buffer
 from ..graph import dispatch
 from .hooks import Arg
-from .kernels import CompiledKernel, kernel_symbol, source_symbol
+from .kernels import CompiledKernel, check_no_spill, kernel_symbol, source_symbol
 from .planner import KernelPlan
 from .shader_factories import make_convolution_shader, make_fft_shader, register_plan_cache
 
 NUM_XCD = 8
 _FUSED_RADICES = {256: (4, 4, 4, 4), 512: (8, 8, 8), 1024: (16, 16, 4)}
-_PHASE_A, _PHASE_B, _PHASE_C, _PHASE_K = 0, 1, 2, 3
-_CTL_ERROR = 512
+_XT_A, _XT_B, _XT_CA, _XT_C = 0, 1, 2, 3
+_XCTL_ERROR = 288
+_XCTL_CNT = 512
 
 
 def _fused_enabled() -> bool:
@@ -59,22 +61,26 @@ def xcd_placement_ok(nwg: int) -> bool:
         return _placement[key]
 
 
-def task_table(npairs: int, na: int, nb: int, nk: int, lag_b: int = 1, lag_c: int = 2) -> np.ndarray:
-    """Per-XCD task queues: words 0..8 are queue offsets, then task words
-    (phase << 30 | pair << 12 | slice).  Each queue starts with the ``nk``
-    kernel-prep tasks of its own Kt copy; pair p runs on queue p % 8; a queue's pairs
-    are software-pipelined: step t issues A(t), B(t - lag_b), C(t - lag_c)."""
-    if not (0 < lag_b < lag_c):
-        raise ValueError("need 0 < lag_b < lag_c")
+def xcd_task_table(npairs: int, ns: int, nb: int, nslot: int) -> np.ndarray:
+    """Per-XCD task queues of the single-launch kernel: words 0..8 are queue offsets,
+    then task words (kind << 30 | local pair << 12 | slice).  Queue q holds pairs
+    q, q + 8, ... (local index i); pair i uses slot i % nslot.  Order: the row tasks
+    of the first nslot pairs, then per group of nslot pairs their tile tasks followed
+    by their row-inverse tasks, which also bring in pair i + nslot (CA) when it
+    exists (C otherwise)."""
     queues: List[List[int]] = []
     for q in range(NUM_XCD):
-        pairs = list(range(q, npairs, NUM_XCD))
-        words: List[int] = [(_PHASE_K << 30) | s for s in range(nk)] if pairs else []
-        for t in range(len(pairs) + lag_c):
-            for phase, lag, n in ((_PHASE_A, 0, na), (_PHASE_B, lag_b, nb), (_PHASE_C, lag_c, na)):
-                i = t - lag
-                if 0 <= i < len(pairs):
-                    words += [(phase << 30) | (pairs[i] << 12) | s for s in range(n)]
+        n = len(range(q, npairs, NUM_XCD))
+        words: List[int] = []
+        for i in range(min(nslot, n)):
+            words += [(_XT_A << 30) | (i << 12) | s for s in range(ns)]
+        for g in range(0, n, nslot):
+            grp = range(g, min(g + nslot, n))
+            for i in grp:
+                words += [(_XT_B << 30) | (i << 12) | c for c in range(nb)]
+            for i in grp:
+                kind = _XT_CA if i + nslot < n else _XT_C
+                words += [(kind << 30) | (i << 12) | s for s in range(ns)]
         queues.append(words)
     offs = np.cumsum([0] + [len(w) for w in queues]).astype(np.uint32)
     return np.concatenate([offs, np.array([w for ws in queues for w in ws], dtype=np.uint32)])
@@ -85,7 +91,8 @@ KPREP_BLOCK = 64   # kernel-prep task: 64 x 64 elements of Kt
 
 
 def _conv_cfg(batch: int, H: int, W: int, normalize: bool, waves: int, rows: int, layout: int = 0,
-              cols_per_wave: int = 1, pairs_per_task: int = 1) -> Tuple[str, KernelPlan]:
+              cols_per_wave: int = 1, pairs_per_task: int = 1, extra: Optional[Dict[str, int]] = None
+              ) -> Tuple[str, KernelPlan]:
     """``struct Cfg`` of the cooperative / phase kernels and the matching KernelPlan
     (radices, twiddle tables)."""
     if H != W or W not in _FUSED_RADICES:
@@ -108,6 +115,7 @@ def _conv_cfg(batch: int, H: int, W: int, normalize: bool, waves: int, rows: int
         f"  static constexpr vkx::u32 PITCH = {pitch}u, KW = {W // 2 + 1}u, NIMG = {batch}u, NPAIRS = {npairs}u;",
         f"  static constexpr vkx::u64 IMG = {H * pitch}ull;",
         f"  static constexpr double SCALE = {scale!r};",
+        *[f"  static constexpr int {k} = {v};" for k, v in (extra or {}).items()],
         f"  using fwd = {pl(plan.fwd)};",
         f"  using inv = {pl(plan.inv)};",
         "};",
@@ -116,13 +124,10 @@ def _conv_cfg(batch: int, H: int, W: int, normalize: bool, waves: int, rows: int
 
 
 def _check_no_spill(kernels: Sequence[CompiledKernel]) -> None:
-    """The pair-packed kernels are used only when they keep everything in registers:
-    a 16-wave variant that spilled to private memory produced wrong rows in its
-    fourth wave slot per SIMD on the MI355X, so any spilling variant is refused."""
+    """Refused at plan time, before any launch (the same check every kernel gets when
+    it is loaded)."""
     for k in kernels:
-        spill = k.code_object_info()["private_bytes"]
-        if spill:
-            raise RuntimeError(f"{k.name}: {spill} bytes/lane of register spills; choose another tiling")
+        check_no_spill(k)
 
 
 def _conv_tu(cfg: str, symbol: str, block: int, waves_per_simd: Optional[int], body: str) -> str:
@@ -132,32 +137,37 @@ def _conv_tu(cfg: str, symbol: str, block: int, waves_per_simd: Optional[int], b
                       f"  {body}", "}", ""])
 
 
-class Conv2DFused:
-    """Single-pass (cooperative) plan for ``batch`` real H x W images (H == W) convolved
-    with one broadcast half spectrum.  ``waves`` waves per workgroup, ``rows`` rows per
-    row task; column tasks take 16 columns; ``wg_per_xcd`` workgroups per XCD."""
+class Conv2DXcd:
+    """Single-launch plan (``conv2d_xcd_body``) for ``batch`` real H x W images (H == W)
+    convolved with one broadcast half spectrum.  ``waves`` waves per workgroup, 16 rows
+    per row task and 16 columns per tile task, ``wg_per_xcd`` workgroups per XCD,
+    ``nslot`` pair spectra of scratch per XCD (2 MiB each at 512 x 512)."""
 
-    def __init__(self, batch: int, H: int, W: int, normalize: bool = True, waves: int = 8, rows: int = 16,
-                 wg_per_xcd: int = 32, lag_b: int = 1, lag_c: int = 2, waves_per_simd: Optional[int] = None):
-        cfg, plan = _conv_cfg(batch, H, W, normalize, waves, rows)
+    ROWS = 16
+
+    def __init__(self, batch: int, H: int, W: int, normalize: bool = True, waves: int = 16, wg_per_xcd: int = 32,
+                 nslot: int = 2, waves_per_simd: Optional[int] = None):
         self.batch, self.H, self.W = int(batch), int(H), int(W)
         self.npairs = (self.batch + 1) // 2
-        self.na = H // rows
+        self.nslot = int(nslot)
+        self.qmax = -(-self.npairs // NUM_XCD)
+        cfg, plan = _conv_cfg(batch, H, W, normalize, waves, self.ROWS,
+                              extra={"NSLOT": self.nslot, "QMAX": self.qmax})
+        self.ns = H // self.ROWS
         self.nb = W // COL_TILE
-        self.nk = (H // KPREP_BLOCK) * (W // KPREP_BLOCK)
-        grid = NUM_XCD * wg_per_xcd
+        grid = NUM_XCD * int(wg_per_xcd)
         self.plan = KernelPlan(**{**plan.__dict__, "NWG": grid})
-        symbol = kernel_symbol("conv2d", self.plan, "fused")
-        src = _conv_tu(cfg, symbol, 64 * waves, waves_per_simd, "vkx::conv2d_pair_body<Cfg>(prm);")
+        symbol = kernel_symbol("conv2d", self.plan, "xcd")
+        src = _conv_tu(cfg, symbol, 64 * waves, waves_per_simd, "vkx::conv2d_xcd_body<Cfg>(prm);")
         args = [Arg("output", "buffer", dt.complex64, 0), Arg("kernel", "buffer", dt.complex64, 1),
                 Arg("scratch", "buffer", dt.complex64, 2), Arg("control", "buffer", dt.uint32, 3),
-                Arg("tasks", "buffer", dt.uint32, 4), Arg("kernel_t", "buffer", dt.complex64, 5)]
+                Arg("tasks", "buffer", dt.uint32, 4)]
         radices = list(self.plan.radices)
         self.kernel = CompiledKernel(src, self.plan, args, radices, radices[::-1],
-                                     name=f"conv2d_fused_{batch}x{H}x{W}")
-        self.table = task_table(self.npairs, self.na, self.nb, self.nk, lag_b, lag_c)
-        self.ctl_words = 1024 + 64 * self.npairs
-        self._dev: Optional[Tuple[Buffer, Buffer, Buffer, Buffer]] = None
+                                     name=f"conv2d_xcd_{batch}x{H}x{W}")
+        self.table = xcd_task_table(self.npairs, self.ns, self.nb, self.nslot)
+        self.ctl_words = _XCTL_CNT + 64 * NUM_XCD * self.qmax
+        self._dev: Optional[Tuple[Buffer, Buffer, Buffer]] = None
         self._checked = False
         self._lock = threading.Lock()
 
@@ -165,41 +175,58 @@ class Conv2DFused:
     def grid(self) -> int:
         return self.plan.NWG
 
+    def scratch_elems(self) -> int:
+        return NUM_XCD * self.nslot * self.H * self.W
+
     def kernels(self) -> List[CompiledKernel]:
         return [self.kernel]
 
-    def resources(self) -> Tuple[Buffer, Buffer, Buffer, Buffer]:
-        """(scratch, control, task table, per-queue Kt); the control block starts at zero
-        and every launch leaves it at zero (error bits aside)."""
+    def resources(self) -> Tuple[Buffer, Buffer, Buffer]:
+        """(scratch, control, task table); the control block starts at zero and every
+        launch leaves it at zero (error bits aside)."""
         if self._dev is None:
             with self._lock:
                 if self._dev is None:
-                    scratch = Buffer((self.npairs * self.H, self.W), dt.complex64)
+                    rt.require_not_capturing("the first call of a 2-D convolution plan")
+                    scratch = Buffer((NUM_XCD * self.nslot * self.H, self.W), dt.complex64)
                     ctl = Buffer((self.ctl_words,), dt.uint32)
                     ctl.write(np.zeros(self.ctl_words, np.uint32))
                     table = Buffer((self.table.size,), dt.uint32)
                     table.write(self.table)
-                    kt = Buffer((NUM_XCD * self.H, self.W), dt.complex64)
-                    self._dev = (scratch, ctl, table, kt)
+                    self._dev = (scratch, ctl, table)
         return self._dev
 
     def values(self, buffer, kernel) -> list:
         if not self._checked:
             _check_no_spill(self.kernels())
             self._checked = True
-        scratch, ctl, table, kt = self.resources()
-        return [buffer, kernel, scratch, ctl, table, kt]
+        scratch, ctl, table = self.resources()
+        return [buffer, kernel, scratch, ctl, table]
 
     def __call__(self, buffer, kernel, graph=None) -> None:
         dispatch(self.kernel, self.values(buffer, kernel), graph)
+        rt.note_async_check(self, rt.current_stream())
+
+    def _error_word(self) -> int:
+        ctl = self.resources()[1]
+        w = np.zeros(1, np.uint32)
+        rt.dtoh(w.ctypes.data, ctl.ptr + 4 * _XCTL_ERROR, 4)
+        return int(w[0])
 
     def error_bits(self) -> int:
-        """Error word of the control block (bit 0: a dependency wait timed out; bit 2: a
-        queue had no workgroup on its XCD and the last workgroup ran it, which is
-        correct but slow).  Synchronises the device."""
-        ctl = self.resources()[1]
-        rt.synchronize()
-        return int(ctl.read(0)[_CTL_ERROR])
+        """Error word of the control block (bit 0: a dependency wait expired, the result
+        is invalid; bit 2: a queue had no workgroup on its XCD and the last workgroup ran
+        it, which is correct but slow).  Synchronises the stream."""
+        rt.check(rt.lib().vkx_stream_synchronize(rt.current_stream()), "vkx_stream_synchronize")
+        rt.forget_async_check(self)
+        return self._error_word()
+
+    def async_error(self):
+        """Message for synchronize() when a launch's dependency wait expired."""
+        if self._error_word() & 1:
+            return (f"2-D convolution of {self.batch} images {self.H}x{self.W}: a dependency wait between "
+                    "workgroups expired; the result is invalid")
+        return None
 
     def clear_error(self) -> None:
         ctl = self.resources()[1]
@@ -252,6 +279,7 @@ class Conv2DPhases:
         if self._dev is None:
             with self._lock:
                 if self._dev is None:
+                    rt.require_not_capturing("the first call of a 2-D convolution plan")
                     self._dev = (Buffer((self.npairs * self.H, self.W), dt.complex64),
                                  Buffer((self.H, self.W), dt.complex64))
         return self._dev
@@ -272,14 +300,21 @@ class Conv2DPhases:
             dispatch(k, vals, graph)
 
 
-def _mode() -> str:
-    """VKX_CONV2D_MODE: "phases" (three kernels, default), "fused" (one cooperative
-    launch), or VKX_FUSED_CONV=0 for the reference's three-dispatch composition."""
-    return os.environ.get("VKX_CONV2D_MODE", "phases")
+def _mode(batch: int, W: int) -> str:
+    """VKX_CONV2D_MODE ("xcd" | "phases") when set, else the measured default (the
+    three phase kernels).  VKX_FUSED_CONV=0 selects the reference's three-dispatch
+    composition instead (see fused_eligible)."""
+    m = os.environ.get("VKX_CONV2D_MODE")
+    if m:
+        if m not in ("xcd", "phases"):
+            raise ValueError("VKX_CONV2D_MODE is 'xcd' or 'phases'")
+        return m
+    return "phases"
 
 
-# cooperative-kernel tilings that keep every value in registers (no private memory)
-_FUSED_TILING = {1024: dict(waves=4, rows=8)}
+# single-launch tilings: one 16-wave workgroup per CU, no register spills at these
+# sizes (1024 would need 128+ registers and spills: it stays on the phase kernels)
+_XCD_TILING = {256: dict(waves=16, wg_per_xcd=32, nslot=2), 512: dict(waves=16, wg_per_xcd=32, nslot=2)}
 # phase-kernel tilings: for 512 the row kernels run one row per wave (16 waves, 64 VGPRs,
 # no spills), so two workgroups per CU hold 8 waves per SIMD while they read HBM
 _PHASE_TILING = {512: dict(waves=16, irow_waves=8)}
@@ -287,8 +322,10 @@ _PHASE_TILING = {512: dict(waves=16, irow_waves=8)}
 
 @lru_cache(maxsize=32)
 def fused_plan(batch: int, H: int, W: int, normalize: bool = True, mode: str = "phases"):
-    if mode == "fused":
-        return Conv2DFused(batch, H, W, normalize, **_FUSED_TILING.get(W, {}))
+    if mode == "xcd":
+        if W not in _XCD_TILING:
+            raise ValueError(f"the single-launch 2-D convolution takes {sorted(_XCD_TILING)} images, not {W}")
+        return Conv2DXcd(batch, H, W, normalize, **_XCD_TILING[W])
     return Conv2DPhases(batch, H, W, normalize, **_PHASE_TILING.get(W, {}))
 
 
@@ -296,9 +333,9 @@ register_plan_cache(fused_plan.cache_clear)
 
 
 def fused_eligible(buffer, kernel, kernel_map=None, buffer_shape=None, transposed_kernel=False,
-                   kernel_inner_only=False, normalize=True, compute_type=None, **_) -> Optional[Conv2DFused]:
-    """The single-pass plan for this convolve2DR call, or None when the call needs the
-    three-dispatch composition."""
+                   kernel_inner_only=False, normalize=True, compute_type=None, **_):
+    """The pair-packed plan (Conv2DXcd or Conv2DPhases) for this convolve2DR call, or
+    None when the call needs the three-dispatch composition."""
     if not _fused_enabled() or kernel_map is not None or transposed_kernel:
         return None
     if compute_type is not None and compute_type is not dt.complex64:
@@ -319,7 +356,7 @@ def fused_eligible(buffer, kernel, kernel_map=None, buffer_shape=None, transpose
         return None
     if batch > 1 and tuple(kernel.shape[-2:]) != (H, wh):
         return None
-    return fused_plan(batch, H, W, bool(normalize), _mode())
+    return fused_plan(batch, H, W, bool(normalize), _mode(batch, W))
 
 
 def try_convolve2DR(buffer, kernel, graph=None, **kw) -> bool:
@@ -344,14 +381,16 @@ def composed_kernels(real_shape, kernel_inner_only=True, compute_type=dt.complex
 
 
 _STANDARD_FUSED = ((64, 512, True), (8, 512, True), (16, 512, True), (6, 512, True), (4, 512, True),
-                   (1, 512, True), (5, 256, True), (4, 256, True), (3, 256, True), (3, 256, False),
-                   (2, 1024, True))
+                   (1, 512, True), (17, 512, True), (5, 256, True), (4, 256, True), (3, 256, True),
+                   (3, 256, False), (40, 256, True), (2, 1024, True))
 
 
 def standard_kernels() -> List:
     ks = list(composed_kernels((64, 512, 512))) + list(composed_kernels((16, 512, 512)))
     for batch, n, norm in _STANDARD_FUSED:
-        for mode in ("phases", "fused"):
+        for mode in ("phases", "xcd"):
+            if mode == "xcd" and n not in _XCD_TILING:
+                continue
             ks += fused_plan(batch, n, n, norm, mode).kernels()
     return ks
 

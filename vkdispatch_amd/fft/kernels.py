@@ -166,6 +166,23 @@ _modules: Dict[str, Tuple[int, int]] = {}
 _mod_lock = threading.Lock()
 
 
+LAUNCH_TAG = "// vkx-launch "
+
+
+def launch_comment(meta: Dict) -> str:
+    import json
+    return LAUNCH_TAG + json.dumps(meta, separators=(",", ":")) + "\n"
+
+
+def parse_launch_comment(source: str) -> Dict:
+    """The launch description a kernel source carries in its first line."""
+    import json
+    first = source.split("\n", 1)[0]
+    if not first.startswith(LAUNCH_TAG):
+        raise ValueError("kernel source has no launch description line")
+    return json.loads(first[len(LAUNCH_TAG):])
+
+
 def load_function(source: str) -> int:
     key = source_key(source)
     with _mod_lock:
@@ -190,15 +207,27 @@ class CompiledKernel:
 
     def __init__(self, source: str, plan: KernelPlan, args, twiddle_fwd: Optional[List[int]],
                  twiddle_inv: Optional[List[int]], name: str = "vkx_fft", roots_n: int = 0):
-        self.source = source
         self.plan = plan
         self.args = list(args)
         self.name = name
         self.tw_fwd = twiddle_fwd
         self.tw_inv = twiddle_inv
         self.roots_n = int(roots_n)
+        # The translation unit describes its own launch in its first line, so a backend
+        # that receives only the source (the reference's stage_compute_plan_create)
+        # can bind the same geometry and twiddle tables (launch_metadata()).
+        self.source = launch_comment(self.launch_metadata(source)) + source
         self._plan_handle = None
         self._lock = threading.Lock()
+
+    def launch_metadata(self, source: Optional[str] = None) -> Dict:
+        return {"symbol": source_symbol(source if source is not None else self.source),
+                "grid": [int(v) for v in self.plan.grid], "block": [int(self.plan.BLOCK), 1, 1],
+                "n": int(self.plan.N), "double": 1 if self.plan.real == "double" else 0,
+                "radices_fwd": [int(r) for r in (self.tw_fwd or [])],
+                "radices_inv": [int(r) for r in (self.tw_inv or [])],
+                "roots_n": self.roots_n,
+                "buffers": [a.role for a in self.args if a.kind == "buffer"]}
 
     def __str__(self):
         return self.source
@@ -211,6 +240,8 @@ class CompiledKernel:
         if self._plan_handle is None:
             with self._lock:
                 if self._plan_handle is None:
+                    check_no_spill(self)
+                    rt.require_not_capturing(f"the first launch of {self.name}")
                     fn = load_function(self.source)
                     d = rt.PlanDesc()
                     d.function = fn
@@ -285,6 +316,22 @@ class CompiledKernel:
         cbuf = ctypes.create_string_buffer(consts, max(1, len(consts)))
         s = rt.current_stream() if stream is None else int(stream)
         rt.check(rt.lib().vkx_plan_exec(h, arr, len(ptrs), cbuf, len(consts), s), f"{self.name} launch")
+
+
+class SpillError(RuntimeError):
+    """A hiprtc kernel needs private (scratch) memory for register spills."""
+
+
+def check_no_spill(kernel: "CompiledKernel") -> None:
+    """Refuse a kernel whose code object spills registers to private memory.  Every
+    kernel the planners emit is built to stay in registers (the tuned plans, the
+    pair-packed convolutions and every FFT shape of the test suite compile without
+    spills); a spilling variant once produced wrong rows on the MI355X with no cause
+    found, so a plan that spills is an error here, never a silent slow path."""
+    info = kernel.code_object_info()
+    if info["private_bytes"]:
+        raise SpillError(f"{kernel.name}: {info['private_bytes']} bytes/lane of register spills "
+                         f"({info['vgprs']} VGPRs); this plan is refused")
 
 
 def code_object_info(code: bytes) -> Dict[str, int]:

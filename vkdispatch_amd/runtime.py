@@ -103,6 +103,7 @@ _SIGS = {
                         ctypes.c_int),
     "vkx_bluestein_b": ([_U64, ctypes.c_int, ctypes.c_int, _U64], ctypes.c_int),
     "vkx_graph_begin": ([_U64], ctypes.c_int),
+    "vkx_stream_is_capturing": ([_U64, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "vkx_graph_end": ([_U64, _PU64], ctypes.c_int),
     "vkx_graph_launch": ([_U64, _U64], ctypes.c_int),
     "vkx_graph_destroy": ([_U64], ctypes.c_int),
@@ -260,6 +261,23 @@ def _raise_async_errors(stream: int) -> None:
         raise RuntimeError("; ".join(errors))
 
 
+def is_capturing(stream=None) -> bool:
+    """True while ``stream`` (default: the current stream) is being captured into a graph."""
+    c = ctypes.c_int(0)
+    check(lib().vkx_stream_is_capturing(current_stream() if stream is None else int(stream), ctypes.byref(c)),
+          "vkx_stream_is_capturing")
+    return c.value != 0
+
+
+def require_not_capturing(what: str) -> None:
+    """Raise if the current stream is capturing: ``what`` allocates or initialises device
+    memory, which a graph capture must not record."""
+    if is_capturing():
+        raise RuntimeError(f"{what} allocates and initialises device memory, which cannot happen while the stream "
+                           "is being captured into a graph: run the same call once before the capture "
+                           "(as a warm-up), then capture")
+
+
 def synchronize(stream=None) -> None:
     s = current_stream() if stream is None else int(stream)
     check(lib().vkx_stream_synchronize(s), "vkx_stream_synchronize")
@@ -272,6 +290,7 @@ def synchronize(stream=None) -> None:
 # ---------------------------------------------------------------------------
 def malloc(nbytes: int) -> int:
     ensure_device()
+    require_not_capturing("a device allocation")
     p = ctypes.c_uint64(0)
     check(lib().vkx_malloc(ctypes.byref(p), max(1, int(nbytes))), "vkx_malloc")
     return p.value
