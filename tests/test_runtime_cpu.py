@@ -121,3 +121,18 @@ def test_graph_record_keeps_plan_scratch_alive():
     g.reset()
     gc.collect()
     assert ref() is None
+
+
+def test_cuda_graph_capture_scope_routes_the_stream_and_restores_it():
+    """The capture scope (vkdispatch/execution_pipeline/cuda_graph_capture.py:22-52) routes
+    this thread's launches to the capturing stream and restores the previous routing."""
+    import vkdispatch_amd as vd
+    assert vd.get_cuda_capture() is None
+    before = rt._state.stream_override
+    with vd.cuda_graph_capture(0x1234) as cap:
+        assert vd.get_cuda_capture() is cap and cap.stream_handle == 0x1234
+        assert rt._state.stream_override == 0x1234
+        with vd.suspend_cuda_capture():
+            assert vd.get_cuda_capture() is None and rt._state.stream_override is None
+        assert vd.get_cuda_capture() is cap and rt._state.stream_override == 0x1234
+    assert vd.get_cuda_capture() is None and rt._state.stream_override == before

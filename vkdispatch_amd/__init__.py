@@ -18,11 +18,12 @@ plan and compiled for gfx950 with hiprtc.
 """
 from . import codegen
 from . import dtypes
-from .buffer import Buffer, RFFTBuffer, asbuffer, asrfftbuffer, from_cuda_array, from_torch
+from .buffer import (Buffer, RFFTBuffer, asbuffer, asrfftbuffer, from_cuda_array, from_torch,
+                     rfftbuffer_from_cuda_array)
 from .context import get_context, initialize, is_cuda, is_dummy, is_vulkan, make_context
 from .dtypes import (bfloat16, complex32, complex64, complex128, complex_bf16, dtype, float16, float32, float64,
                     int16, int32, int64, uint16, uint32, uint64)
-from .graph import CommandGraph, global_graph
+from .graph import CommandGraph, cuda_graph_capture, get_cuda_capture, global_graph, suspend_cuda_capture
 from .runtime import Event, set_stream, stream_scope, synchronize
 from .shader_map import MappingFunction, map
 from . import fft

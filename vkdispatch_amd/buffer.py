@@ -251,11 +251,11 @@ class RFFTBuffer(Buffer):
     """Buffer of shape ``shape[:-1] + (shape[-1]//2 + 1,)`` complex values holding either a
     real signal of ``real_shape`` (rows of pitch 2*(N/2+1) reals) or its half spectrum."""
 
-    def __init__(self, shape, fourier_type=complex64):
+    def __init__(self, shape, fourier_type=complex64, external=None):
         if not dtypes.is_complex(fourier_type):
             raise ValueError("RFFTBuffer fourier_type must be complex32, complex64, or complex128")
         shape = tuple(int(s) for s in shape)
-        super().__init__(tuple(shape[:-1]) + (shape[-1] // 2 + 1,), fourier_type)
+        super().__init__(tuple(shape[:-1]) + (shape[-1] // 2 + 1,), fourier_type, external=external)
         self.real_shape = shape
         self.fourier_shape = self.shape
         self.real_type = fourier_type.child_type
@@ -298,6 +298,30 @@ class RFFTBuffer(Buffer):
             self.write(data, index)
         else:
             self.write(np.ascontiguousarray(data.astype(self.var_type.numpy)), index)
+
+
+def rfftbuffer_from_cuda_array(obj, real_shape, keepalive=True) -> RFFTBuffer:
+    """Alias a device array (torch-ROCm tensor, any ``__cuda_array_interface__`` object)
+    as an RFFTBuffer of ``real_shape`` without a copy: a complex array of the half
+    spectrum shape ``real_shape[:-1] + (N//2 + 1,)``, or a real array of the packed
+    layout ``real_shape[:-1] + (2 * (N//2 + 1),)`` (the reference's rows of pitch
+    2(N/2+1) reals, vkdispatch/base/buffer.py:377-448)."""
+    if not hasattr(obj, "__cuda_array_interface__"):
+        raise TypeError("Expected an object with __cuda_array_interface__")
+    rt.ensure_device()
+    real_shape = tuple(int(s) for s in real_shape)
+    fshape = real_shape[:-1] + (real_shape[-1] // 2 + 1,)
+    shape, vt, ext = _cai_external(obj, None, True, None, keepalive)
+    if dtypes.is_complex(vt):
+        if shape != fshape:
+            raise ValueError(f"a complex alias must have the half-spectrum shape {fshape}, got {shape}")
+        ftype = vt
+    else:
+        if shape != real_shape[:-1] + (2 * fshape[-1],):
+            raise ValueError(f"a real alias must have the packed shape {real_shape[:-1] + (2 * fshape[-1],)}, "
+                             f"got {shape}")
+        ftype = dtypes.complex_from_float(vt)
+    return RFFTBuffer(real_shape, ftype, external=ext)
 
 
 def asrfftbuffer(data, fourier_type=None) -> RFFTBuffer:

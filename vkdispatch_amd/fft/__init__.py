@@ -13,4 +13,4 @@ from .src_functions import (convolve2D_print_src, convolve2D_src, convolve2DR_pr
                             fft_print_src, fft_src, ifft2_print_src, ifft2_src, ifft3_print_src, ifft3_src,
                             ifft_print_src, ifft_src, irfft2_print_src, irfft2_src, irfft3_print_src, irfft3_src,
                             irfft_print_src, irfft_src, rfft2_print_src, rfft2_src, rfft3_print_src, rfft3_src,
-                            rfft_print_src, rfft_src, transpose_src)
+                            rfft_print_src, rfft_src, transpose_print_src, transpose_src)

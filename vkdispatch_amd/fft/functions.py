@@ -373,7 +373,7 @@ def convolve2DR(buffer: RFFTBuffer, kernel: RFFTBuffer, kernel_map: MappingFunct
     from . import fused
     if fused.try_convolve2DR(buffer, kernel, kernel_map=kernel_map, buffer_shape=buffer_shape,
                              transposed_kernel=transposed_kernel, kernel_inner_only=kernel_inner_only, graph=graph,
-                             normalize=normalize, compute_type=compute_type):
+                             normalize=normalize, compute_type=compute_type, print_shader=print_shader):
         return
     rfft(buffer, graph=graph, print_shader=print_shader, compute_type=compute_type)
     convolve(buffer, kernel, kernel_map=kernel_map, buffer_shape=buffer_shape, graph=graph,

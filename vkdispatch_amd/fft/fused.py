@@ -332,37 +332,52 @@ def fused_plan(batch: int, H: int, W: int, normalize: bool = True, mode: str = "
 register_plan_cache(fused_plan.cache_clear)
 
 
-def fused_eligible(buffer, kernel, kernel_map=None, buffer_shape=None, transposed_kernel=False,
-                   kernel_inner_only=False, normalize=True, compute_type=None, **_):
-    """The pair-packed plan (Conv2DXcd or Conv2DPhases) for this convolve2DR call, or
-    None when the call needs the three-dispatch composition."""
+def plan_for_shape(real_shape, kernel_map=None, transposed_kernel=False, kernel_inner_only=False, normalize=True,
+                   compute_type=None):
+    """The pair-packed plan for a convolve2DR over complex64 RFFTBuffers of ``real_shape``
+    with a kernel spectrum of one image, or None when the call takes the reference's
+    three-dispatch composition."""
     if not _fused_enabled() or kernel_map is not None or transposed_kernel:
         return None
     if compute_type is not None and compute_type is not dt.complex64:
         return None
+    rs = tuple(real_shape)
+    if len(rs) not in (2, 3):
+        return None
+    H, W = rs[-2], rs[-1]
+    batch = rs[0] if len(rs) == 3 else 1
+    if H != W or W not in _FUSED_RADICES:
+        return None
+    if batch > 1 and not kernel_inner_only:
+        return None  # the reference then reads one kernel spectrum per image
+    return fused_plan(batch, H, W, bool(normalize), _mode(batch, W))
+
+
+def fused_eligible(buffer, kernel, kernel_map=None, buffer_shape=None, transposed_kernel=False,
+                   kernel_inner_only=False, normalize=True, compute_type=None, **_):
+    """The pair-packed plan (Conv2DXcd or Conv2DPhases) for this convolve2DR call, or
+    None when the call needs the three-dispatch composition."""
     if getattr(buffer, "var_type", None) is not dt.complex64 or getattr(kernel, "var_type", None) is not dt.complex64:
         return None
     rs = tuple(getattr(buffer, "real_shape", ()))
     if len(rs) not in (2, 3) or (buffer_shape is not None and tuple(buffer_shape) != tuple(buffer.shape)):
         return None
     H, W = rs[-2], rs[-1]
-    batch = rs[0] if len(rs) == 3 else 1
-    if H != W or W not in _FUSED_RADICES:
-        return None
     wh = W // 2 + 1
-    if batch > 1 and not kernel_inner_only:
-        return None  # the reference then reads one kernel spectrum per image
     if kernel.size < H * wh:
         return None
-    if batch > 1 and tuple(kernel.shape[-2:]) != (H, wh):
+    if len(rs) == 3 and rs[0] > 1 and tuple(kernel.shape[-2:]) != (H, wh):
         return None
-    return fused_plan(batch, H, W, bool(normalize), _mode(batch, W))
+    return plan_for_shape(rs, kernel_map, transposed_kernel, kernel_inner_only, normalize, compute_type)
 
 
-def try_convolve2DR(buffer, kernel, graph=None, **kw) -> bool:
+def try_convolve2DR(buffer, kernel, graph=None, print_shader=False, **kw) -> bool:
     plan = fused_eligible(buffer, kernel, **kw)
     if plan is None:
         return False
+    if print_shader:
+        for k in plan.kernels():
+            print(k.source)
     plan(buffer, kernel, graph)
     return True
 

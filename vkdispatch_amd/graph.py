@@ -10,6 +10,7 @@ instantiated graph on every later submit.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes
 import threading
 from typing import List, Optional, Tuple
@@ -79,9 +80,17 @@ class CommandGraph:
                use_graph: bool = True) -> None:
         if not self._records:
             return
+        cap = get_cuda_capture()
+        if cap is not None and stream is None:
+            stream = cap.stream_handle
         s = rt.current_stream() if stream is None else int(stream)
         reps = 1 if instance_count is None else int(instance_count)
-        if use_graph:
+        if cap is not None or rt.is_capturing(s):
+            # the caller is capturing this stream (torch.cuda.graph): launch the records
+            # directly so that they become nodes of the caller's graph
+            for _ in range(reps):
+                self._replay(s)
+        elif use_graph:
             ex = self.capture(s)
             for _ in range(reps):
                 rt.check(rt.lib().vkx_graph_launch(ex, s), "vkx_graph_launch")
@@ -106,6 +115,66 @@ class CommandGraph:
 
 
 _tls = threading.local()
+
+
+# ---------------------------------------------------------------------------
+# capture scope for a caller's graph (torch.cuda.graph / hipGraph), the
+# reference's vkdispatch/execution_pipeline/cuda_graph_capture.py:9-52
+# ---------------------------------------------------------------------------
+class CUDAGraphCapture:
+    """State of an open capture scope: the stream vd launches go to, and the uniform
+    buffers the reference would keep alive (unused here: Var / Const values travel in
+    the kernel's by-value argument, which the captured launch copies)."""
+
+    def __init__(self, cuda_stream):
+        self.cuda_stream = cuda_stream
+        self.uniform_buffers: List[object] = []
+        if cuda_stream is None:
+            import torch
+            cuda_stream = torch.cuda.current_stream()
+        self.stream_handle = int(cuda_stream.cuda_stream) if hasattr(cuda_stream, "cuda_stream") else int(cuda_stream)
+
+    def add_uniform_buffer(self, buffer) -> None:
+        self.uniform_buffers.append(buffer)
+
+
+def get_cuda_capture() -> Optional[CUDAGraphCapture]:
+    return getattr(_tls, "capture", None)
+
+
+@contextlib.contextmanager
+def cuda_graph_capture(cuda_stream=None):
+    """Inside ``with torch.cuda.graph(g): with vd.cuda_graph_capture(torch.cuda.current_stream()):``
+    every vd launch (direct calls and CommandGraph.submit) goes to ``cuda_stream`` (default:
+    torch's current stream) and is recorded into the caller's graph.  Plans must have been
+    used once before the capture (their first use allocates device memory)."""
+    cap = CUDAGraphCapture(cuda_stream)
+    prev_cap = get_cuda_capture()
+    prev_stream = rt._state.stream_override
+    _tls.capture = cap
+    rt._state.stream_override = cap.stream_handle
+    try:
+        yield cap
+    finally:
+        _tls.capture = prev_cap
+        rt._state.stream_override = prev_stream
+
+
+@contextlib.contextmanager
+def suspend_cuda_capture():
+    """Temporarily leave the capture scope (for work that must not be captured)."""
+    cap = get_cuda_capture()
+    if cap is None:
+        yield
+        return
+    prev_stream = rt._state.stream_override
+    _tls.capture = None
+    rt._state.stream_override = None
+    try:
+        yield
+    finally:
+        _tls.capture = cap
+        rt._state.stream_override = prev_stream
 
 
 def global_graph() -> CommandGraph:
