@@ -271,7 +271,8 @@ def _conv_kernel_bytes(plan, batch, n):
     real_b = batch * n * n * 4
     kb = n * wh * 8
     if isinstance(plan, fused.Conv2DXcd):
-        return [2 * real_b + kb]
+        kt_b = n * n * 8
+        return [kb + kt_b, 2 * real_b + kt_b]
     cplx_b = (batch + 1) // 2 * n * n * 8
     kt_b = n * n * 8
     return [real_b + cplx_b + kb + kt_b, 2 * cplx_b + kt_b, cplx_b + real_b]

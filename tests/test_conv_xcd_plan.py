@@ -152,39 +152,41 @@ def test_xcd_queues_drain_without_hazards(npairs, wg):
 
 @pytest.mark.parametrize("seed", range(3))
 def test_xcd_queues_full_size_emulation(seed):
-    """The 64 x 512^2 workload's queues (32 row slices, 32 tiles per pair) with 32
-    workgroups per XCD, as launched."""
-    _emulate(32, ns=32, nb=32, nslot=2, wg_per_queue=32, seed=seed)
+    """The 64 x 512^2 workload's queues with the shipped tiling (64 row slices, 64 tiles
+    per pair, 128 workgroups per XCD), as launched."""
+    cfg = fused._XCD_TILING[512]
+    _emulate(32, ns=512 // cfg["rows"], nb=512 // cfg["ct"], nslot=cfg["nslot"], wg_per_queue=cfg["wg_per_xcd"],
+             seed=seed)
 
 
 def test_xcd_task_words_fit_their_fields():
-    t = fused.xcd_task_table(32, 32, 32, 2)
+    t = fused.xcd_task_table(32, 64, 64, 2)
     q = _queues(t)
-    # per pair: 32 row tasks, 32 tiles, 32 row-inverse tasks, where the row tasks of all
+    # per pair: 64 row tasks, 64 tiles, 64 row-inverse tasks, where the row tasks of all
     # but the first two pairs of a queue ride in the CA tasks of the pair before
-    assert sum(len(x) for x in q) == 32 * 32 * 3 - 16 * 32
+    assert sum(len(x) for x in q) == 32 * 64 * 3 - 16 * 64
     for tasks in q:
         for w in tasks:
             kind, i, s = _decode(w)
-            assert kind in (XT_A, XT_B, XT_CA, XT_C) and i < 4 and s < 32
+            assert kind in (XT_A, XT_B, XT_CA, XT_C) and i < 4 and s < 64
 
 
 # ---------------------------------------------------------------------------
 # index bounds of every access, with the kernel's formulas
 # ---------------------------------------------------------------------------
-def _tile_index(H, ky, kx):
-    return (kx >> 4) * (16 * H) + ky * 16 + (kx & 15)
+def _tile_index(H, ky, kx, ct):
+    return (kx // ct) * (ct * H) + ky * ct + (kx % ct)
 
 
 @pytest.mark.parametrize("batch", [64, 15, 16, 17, 1, 40])
 @pytest.mark.parametrize("H", sorted(fused._XCD_TILING))
 def test_xcd_addresses_stay_in_bounds(batch, H):
     W = H
-    assert fused._XCD_TILING[H]["waves"] == 16 and fused._XCD_TILING[H]["nslot"] == 2
-    nslot, ns, nb, rows = 2, H // 16, W // 16, 16
+    cfg = fused._XCD_TILING[H]
+    nw, rows, ct, nslot = cfg["waves"], cfg["rows"], cfg["ct"], cfg["nslot"]
+    ns, nb = H // rows, W // ct
     npairs = (batch + 1) // 2
     qmax = -(-npairs // 8)
-    nw = 16
     pitch, img, kw = 2 * (W // 2 + 1), H * 2 * (W // 2 + 1), W // 2 + 1
     scratch_elems = 8 * nslot * H * W
     image_floats = batch * img
@@ -192,6 +194,8 @@ def test_xcd_addresses_stay_in_bounds(batch, H):
     table = fused.xcd_task_table(npairs, ns, nb, nslot)
     lanes = np.arange(64)
     tids = np.arange(64 * nw)
+    np_ = W + W // 32 + 1
+    lds_elems = max(ct, nw) * np_
     for q, tasks in enumerate(_queues(table)):
         for w in tasks:
             kind, i, s = _decode(w)
@@ -203,35 +207,46 @@ def test_xcd_addresses_stay_in_bounds(batch, H):
                 assert ii < qmax
                 assert 512 + 64 * (q * qmax + ii) + 32 < ctl_words
             if kind in (XT_A, XT_CA, XT_C):
+                assert s < ns
                 for wave in range(nw):
-                    y = s * rows + wave
-                    assert y < H
-                    kx = lanes[:, None] + 64 * np.arange(W // 64)[None, :]
-                    idx = base + _tile_index(H, y, kx)
-                    assert idx.min() >= 0 and idx.max() < scratch_elems
-                    for p in ([pair, pair + 8 * nslot] if kind == XT_CA else [pair]):
-                        assert p < npairs
-                        n = kx
-                        fa = 2 * p * img + y * pitch + n
-                        assert fa.max() < image_floats
-                        if 2 * p + 1 < batch:
-                            assert (fa + img).max() < image_floats
+                    for j in range(rows // nw):
+                        y = s * rows + wave + nw * j
+                        assert y < H
+                        kx = lanes[:, None] + 64 * np.arange(W // 64)[None, :]
+                        idx = base + _tile_index(H, y, kx, ct)
+                        assert idx.min() >= 0 and idx.max() < scratch_elems
+                        for p in ([pair, pair + 8 * nslot] if kind == XT_CA else [pair]):
+                            assert p < npairs
+                            fa = 2 * p * img + y * pitch + kx
+                            assert fa.max() < image_floats
+                            if 2 * p + 1 < batch:
+                                assert (fa + img).max() < image_floats
+                    assert wave * np_ + np_ <= lds_elems  # row exchange region
             else:
                 assert s < nb
-                e = tids[:, None] + np.arange(16 * H // (64 * nw))[None, :] * 64 * nw
-                idx = base + s * 16 * H + e
+                e = tids[:, None] + np.arange(ct * H // (64 * nw))[None, :] * 64 * nw
+                idx = base + s * ct * H + e
                 assert idx.max() < scratch_elems
-                ky, kx = e >> 4, s * 16 + (e & 15)
-                kym = (H - ky) & (H - 1)
-                lo = kx <= W // 2
-                kidx = np.where(lo, ky * kw + kx, kym * kw + (W - kx))
-                assert kidx.min() >= 0 and kidx.max() < H * kw
-                edge = (kx == 0) | (kx == W // 2)
-                if edge.any():
-                    assert (kym * kw + kx)[edge].max() < H * kw
-                # LDS: column-major tile with a pad every 32 rows
-                np_ = W + W // 32 + 1
-                assert ((e & 15) * np_ + ky + (ky >> 5)).max() < 16 * np_
+                ky, c = e // ct, e % ct
+                assert (c * np_ + ky + (ky >> 5)).max() < lds_elems
+                # Kt columns of this tile (column-major H x W)
+                kx = s * ct + np.arange(ct)
+                assert (kx[:, None] * H + np.arange(H)[None, :]).max() < H * W
+
+
+def test_kt_prep_covers_every_element_once():
+    """conv2d_kt_body: block b (64 x 64) of the column-major Kt, one per workgroup
+    (conv_kprep's index formulas), writes every element of H x W exactly once."""
+    for H in sorted(fused._XCD_TILING):
+        W, KB, nt = H, 64, 64 * fused._XCD_TILING[H]["waves"]
+        hits = np.zeros(H * W, np.int32)
+        for b in range((H // KB) * (W // KB)):
+            ky0, kx0 = (b // (W // KB)) * KB, (b % (W // KB)) * KB
+            for i in range(KB * KB // nt):
+                e = np.arange(nt) + i * nt
+                ky, kx = ky0 + e % KB, kx0 + e // KB
+                np.add.at(hits, kx * H + ky, 1)
+        assert (hits == 1).all()
 
 
 # ---------------------------------------------------------------------------
@@ -258,13 +273,15 @@ def test_xcd_dataflow_in_numpy_matches_oracle():
     rng = np.random.default_rng(3)
     batch, H = 21, 64          # 11 pairs: queues 0..2 hold two pairs, the rest one
     W = H
-    ns, nb, nslot, rows = H // 16, W // 16, 2, 16
+    cfg = fused._XCD_TILING[512]
+    rows, ct, nslot = cfg["rows"], cfg["ct"], cfg["nslot"]
+    ns, nb = H // rows, W // ct
     x = rng.uniform(-1, 1, (batch, H, W))
     K = (rng.standard_normal((H, W // 2 + 1)) + 1j * rng.standard_normal((H, W // 2 + 1)))
     Kt = _kt(K, H, W, 1.0 / (H * W))
     npairs = (batch + 1) // 2
     out = np.zeros_like(x)
-    slots = np.zeros((8, nslot, W // 16, H, 16), np.complex128)  # tile-major: [tile][ky][kx%16]
+    slots = np.zeros((8, nslot, W // ct, H, ct), np.complex128)  # tile-major: [tile][ky][kx % ct]
 
     def rows_fwd(q, i, s):
         p = q + 8 * i
@@ -272,17 +289,17 @@ def test_xcd_dataflow_in_numpy_matches_oracle():
             z = x[2 * p, y] + 1j * (x[2 * p + 1, y] if 2 * p + 1 < batch else 0)
             Z = np.fft.fft(z)
             for kx in range(W):
-                slots[q, i % nslot, kx >> 4, y, kx & 15] = Z[kx]
+                slots[q, i % nslot, kx // ct, y, kx % ct] = Z[kx]
 
     def tile(q, i, c):
         T = slots[q, i % nslot, c]          # (H, 16)
-        F = np.fft.fft(T, axis=0) * Kt[:, c * 16:(c + 1) * 16]
+        F = np.fft.fft(T, axis=0) * Kt[:, c * ct:(c + 1) * ct]
         slots[q, i % nslot, c] = np.fft.ifft(F, axis=0) * H
 
     def rows_inv(q, i, s):
         p = q + 8 * i
         for y in range(s * rows, (s + 1) * rows):
-            Z = np.array([slots[q, i % nslot, kx >> 4, y, kx & 15] for kx in range(W)])
+            Z = np.array([slots[q, i % nslot, kx // ct, y, kx % ct] for kx in range(W)])
             z = np.fft.ifft(Z) * W
             out[2 * p, y] = z.real
             if 2 * p + 1 < batch:

@@ -273,3 +273,28 @@ def test_command_graph_replay():
     assert_close(buf.read(0), O.dft(x))
     g2.submit()
     assert_close(buf.read(0), O.dft(O.dft(x)), rel=1e-5)
+
+
+def test_const_hook_arguments_run_on_the_gpu():
+    """Const[...] hook arguments (vkdispatch/codegen/arguments.py:10-20) carry their value
+    into the kernel like Var: a complex constant multiplies the spectrum and an integer
+    constant shifts the kernel-buffer index (X[k] * w * K[(k + s) mod N])."""
+    rng = np.random.default_rng(55)
+    N, B = 512, 3
+    x = (rng.standard_normal((B, N)) + 1j * rng.standard_normal((B, N))).astype(np.complex64)
+    kv = (rng.standard_normal(N) + 1j * rng.standard_normal(N)).astype(np.complex64)
+    buf = vd.asbuffer(x)
+    kbuf = vd.asbuffer(kv)
+
+    @vd.map
+    def km(kern: vc.Buffer[vc.c64], shift: vc.Const[vc.u32], w: vc.Const[vc.c64]):
+        op = vd.fft.read_op()
+        kk = vc.new_register(vc.c64)
+        op.read_from_buffer(kern, register=kk, io_index=(op.fft_index + shift) % N)
+        op.register[:] = vc.mult_complex(vc.mult_complex(op.register, w), kk)
+
+    w = 0.5 - 0.25j
+    vd.fft.convolve(buf, kbuf, np.uint32(7), w, kernel_map=km)
+    X = np.fft.fft(x.astype(np.complex128), axis=-1)
+    ref = np.fft.ifft(X * w * np.roll(kv.astype(np.complex128), -7)[None, :], axis=-1)
+    assert_close(buf.read(0), ref)

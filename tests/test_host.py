@@ -292,3 +292,23 @@ def test_tuned_plans_do_not_spill():
     K.precompile([k for _, k in built])
     for key, k in built:
         assert k.code_object_info()["private_bytes"] == 0, (key, k.code_object_info())
+
+
+def test_vkfft_zero_blocks_map_to_read_ranges_and_compile(tmp_path, monkeypatch):
+    """vd.vkfft padding (VkFFT's zero block [left, right) per axis): blocks at an end of the
+    axis become a kept range, interior blocks a wrapped kept range; both compile."""
+    from vkdispatch_amd import vkfft
+    from vkdispatch_amd.fft.shader_factories import ZeroBlock, _resolve_signal_range
+    assert vkfft._signal_range(None, 32) is None and vkfft._signal_range((5, 5), 32) is None
+    assert vkfft._signal_range((24, 32), 32) == (0, 24)
+    assert vkfft._signal_range((0, 8), 32) == (8, 32)
+    zb = vkfft._signal_range((8, 16), 32)
+    assert isinstance(zb, ZeroBlock) and _resolve_signal_range(zb, 32) == (16, 8)
+    with pytest.raises(ValueError):
+        vkfft._signal_range((8, 40), 32)
+    with pytest.raises(ValueError):
+        _resolve_signal_range((16, 8), 32)  # the public argument stays an ordered range
+    monkeypatch.setenv("VKX_KERNEL_CACHE", str(tmp_path))
+    k = vd.fft.make_convolution_shader((3, 32, 32), None, 1, 1, input_signal_range=zb)
+    assert "SIG_LO = 16, SIG_HI = 8" in k.source
+    assert K.compile_source(k.source)[:4] == b"\x7fELF"

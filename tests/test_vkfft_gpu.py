@@ -122,7 +122,16 @@ def test_convolve2D_zero_padding_ranges():
     xz[:, 24:, :] = 0
     xz[:, :, :8] = 0
     assert O.rel_l2(buf.read(0), O.vkfft_convolve2D(xz, K, normalize=True)) <= 1e-5
-    with pytest.raises(NotImplementedError):
-        vd.vkfft.convolve2D(vd.asbuffer(x), kb, padding=((8, 16), (0, 0)))
+    # interior blocks on either axis and on both (VkFFT's fft_zeropad_left/right with
+    # 0 < left < right < N): the block is read as zeros
+    for pad in (((8, 16), (0, 0)), ((0, 0), (5, 27)), ((3, 29), (10, 11)), ((1, 31), (16, 32))):
+        buf = vd.asbuffer(x)
+        vd.vkfft.convolve2D(buf, kb, normalize=True, padding=pad)
+        xz = x.astype(np.complex128).copy()
+        xz[:, pad[0][0]:pad[0][1], :] = 0
+        xz[:, :, pad[1][0]:pad[1][1]] = 0
+        assert O.rel_l2(buf.read(0), O.vkfft_convolve2D(xz, K, normalize=True)) <= 1e-5, pad
+    with pytest.raises(ValueError):
+        vd.vkfft.convolve2D(vd.asbuffer(x), kb, padding=((8, 40), (0, 0)))
     with pytest.raises(ValueError):
         vd.vkfft.convolve2D(vd.asbuffer(x), vd.asbuffer(crand(7, 7)))

@@ -467,20 +467,25 @@ __device__ __forceinline__ void conv2d_phase_body(const KParams& prm) {
 // register), so every value a task reads from scratch was written on the same XCD
 // and is found in that XCD's L2 (the transposes never leave the XCD unless the L2
 // evicts them).  Pair i of queue q is pair q + 8 i and lives in slot i % NSLOT.
+// Slot images are tile-major with CT-column tiles: element (ky, kx) at
+// (kx / CT) CT H + ky CT + kx % CT, so a row task writes or reads CT-element runs
+// from registers and a column tile is one contiguous block.
 //
-// Tasks (slice s = RA rows, tile c = 16 columns):
-//   A(i, s)     row FFTs of pair i, rows of slice s -> slot (tile-major layout)
-//   B(i, c)     column FFT . Kt . column IFFT of tile c, in place in the slot;
-//               Kt is built from the half spectrum K in LDS by the task itself
-//   CA(i, s)    row IFFTs of pair i, slice s -> images, then A(i + NSLOT, s):
-//               the next pair of the slot enters the rows pair i just left
+// Tasks (slice s = RA rows, tile c = CT columns):
+//   A(i, s)     row FFTs of pair i, rows of slice s -> slot
+//   B(i, c)     column FFT . Kt . column IFFT of tile c, in place in the slot; Kt is
+//               the column-major conj(Kf) SCALE prepared by conv2d_kt_body
+//   CA(i, s)    row IFFTs of pair i, slice s -> images, then A(i + NSLOT, s): the
+//               next pair of the slot enters the rows pair i just left
 //   C(i, s)     row IFFTs only (the last NSLOT pairs of a queue)
 // A queue lists A(0..NSLOT-1), then per group of NSLOT pairs all their B tasks,
 // then all their CA / C tasks.  B(i) waits for every row slice of pair i (the A or
 // CA tasks that wrote it); CA(i) / C(i) wait for every tile of B(i).  A task waits
 // only for tasks earlier in its queue, which running workgroups already hold, so
 // the launch cannot deadlock; every wait is bounded (an expired wait sets error
-// bit 0 and the task goes on; the host reports it).
+// bit 0 and the task goes on; the host reports it).  Several small workgroups per
+// CU run unrelated tasks side by side, so one's memory waits overlap another's
+// arithmetic.
 //
 // Memory ordering inside one XCD: a task's stores complete to L2 (s_waitcnt) before
 // its workgroup bumps the done counter; a waiting workgroup polls the counter, then
@@ -497,27 +502,37 @@ constexpr u32 XCTL_TICKET = 0;  // + 32 q: next ticket of queue q
 constexpr u32 XCTL_EXIT = 256;  // workgroups that have left
 constexpr u32 XCTL_ERROR = 288;  // bit 0: a wait expired; bit 2: the last workgroup served a queue
 constexpr u32 XCTL_CNT = 512;   // + 64 (q QMAX + i): rows of pair i done at +0, tiles done at +32
-constexpr u32 XCTL_WORDS_FIXED = 512;
 
 template <class C> struct Xcd {
   using WC = WaveCfg<C::W, C::NP>;
-  static constexpr int NV = C::W / 64;     // points per lane of one transform
+  static constexpr int CT = C::CT;           // columns per tile
+  static constexpr int NV = C::W / 64;       // points per lane of one transform
   static constexpr int RPW = C::RA / C::NW;  // rows per wave in a row task
   static constexpr int NT = C::NW * 64;
-  static constexpr int CB = 16 / C::NW;    // columns per wave in a tile task
-  static constexpr u32 NS = C::H / C::RA;  // row slices per pair
-  static constexpr u32 NB = C::W / 16;     // column tiles per pair
+  static constexpr int CB = CT / C::NW;      // columns per wave in a tile task
+  static constexpr u32 NS = C::H / C::RA;    // row slices per pair
+  static constexpr u32 NB = C::W / CT;       // column tiles per pair
   static constexpr u64 PAIR = (u64)C::H * (u64)C::W;
-  static constexpr int IT = 16 * C::H / NT;  // tile elements per thread
-  static constexpr int LDS = 16 * C::NP;     // one tile, column-major, padded
+  static constexpr int IT = CT * C::H / NT;  // tile elements per thread
+  static constexpr int LDS = (CT > C::NW ? CT : C::NW) * C::NP;
   static_assert(C::H == C::W && C::W % 64 == 0, "square images, rows a multiple of 64");
-  static_assert(C::RA % C::NW == 0 && 16 % C::NW == 0 && (16 * C::H) % NT == 0, "task split");
-  static_assert(C::NW * C::NP <= LDS, "row exchange regions fit the tile area");
+  static_assert(C::RA % C::NW == 0 && CT % C::NW == 0 && (CT * C::H) % NT == 0, "task split");
+  static_assert(64 % CT == 0 && (CT & (CT - 1)) == 0, "tile width a power of two dividing a wave");
   static_assert(C::NP >= C::H + C::H / 32, "LDS column region");
 };
 
+template <class C> __device__ __forceinline__ u32 xcd_tile_index(u32 ky, u32 kx) {
+  return (kx / (u32)C::CT) * (u32)(C::CT * C::H) + ky * (u32)C::CT + (kx % (u32)C::CT);
+}
+
 template <class C> __device__ __forceinline__ u32* xcd_cnt(u32* ctl, u32 q, u32 i) {
   return ctl + XCTL_CNT + 64u * (q * (u32)C::QMAX + i);
+}
+
+// Kt for the single launch: block `slice` (64 x 64) of conj(Kf) SCALE, column-major
+// (kt[kx H + ky]); one workgroup per block
+template <class C> __device__ __forceinline__ void conv2d_kt_body(const KParams& prm) {
+  conv_kprep<C>(bufp<const cpx<float>>(prm, 1), bufp<cpx<float>>(prm, 5), blockIdx.x, (int)threadIdx.x);
 }
 
 // the RPW rows of this wave (slice rows y0 + wave + NW j), in the first-pass layout
@@ -540,7 +555,7 @@ __device__ __forceinline__ void xcd_load_rows(const float* x, u32 pair, u32 slic
   }
 }
 
-// row FFTs of z = x_a + i x_b into the slot (tile-major: four 128-byte lines per store)
+// row FFTs of z = x_a + i x_b into the slot
 template <class C, class TS>
 __device__ __forceinline__ void xcd_rows_fwd(const TS& ts, cpx<float>* S, u32 slice, cpx<float>* xch, int wave,
                                              int lane, const float (&ra)[Xcd<C>::RPW][Xcd<C>::NV],
@@ -554,7 +569,9 @@ __device__ __forceinline__ void xcd_rows_fwd(const TS& ts, cpx<float>* S, u32 sl
 #pragma unroll
     for (int k = 0; k < X::NV; ++k) v[k] = cpx<float>{ra[j][k], rb[j][k]};
     ts.template run<-1>(v, xch, lane, 0);
-    for_layout<typename X::WC, FL::R>(lane, [&](auto slot, u32 kx) { S[tile_index<C>(y, kx)] = v[decltype(slot)::value]; });
+    for_layout<typename X::WC, FL::R>(lane, [&](auto slot, u32 kx) {
+      S[xcd_tile_index<C>(y, kx)] = v[decltype(slot)::value];
+    });
   }
 }
 
@@ -571,7 +588,7 @@ __device__ __forceinline__ void xcd_rows_inv(const TS& ts, const cpx<float>* S, 
   for (int j = 0; j < X::RPW; ++j) {  // every load of the task in flight first
     const u32 y = slice * C::RA + (u32)(wave + C::NW * j);
     for_layout<typename X::WC, I0::R>(lane, [&](auto slot, u32 kx) {
-      v[j][decltype(slot)::value] = ld_c64_l2(S + tile_index<C>(y, kx));
+      v[j][decltype(slot)::value] = ld_c64_l2(S + xcd_tile_index<C>(y, kx));
     });
   }
 #pragma unroll
@@ -588,104 +605,58 @@ __device__ __forceinline__ void xcd_rows_inv(const TS& ts, const cpx<float>* S, 
   }
 }
 
-// Kt(ky, kx) = conj(Kf[ky][kx]) * SCALE with Kf the full spectrum rebuilt from the
-// half spectrum K (rules at the top of this file).  EDGE: the tile holds column 0 or
-// W/2, whose value averages two entries of K.
-template <class C, bool EDGE> __device__ __forceinline__ cpx<float> xcd_kt(const cpx<float>* K, u32 ky, u32 kx) {
-  const u32 kym = (C::H - ky) & (C::H - 1);
-  const bool lo = kx <= (u32)(C::W / 2);
-  cpx<float> kf = K[lo ? ky * C::KW + kx : kym * C::KW + (C::W - kx)];
-  if (!lo) kf.y = -kf.y;
-  if constexpr (EDGE) {
-    if (kx == 0 || kx == (u32)(C::W / 2)) {
-      const cpx<float> k1 = K[kym * C::KW + kx];
-      kf = cpx<float>{0.5f * (kf.x + k1.x), 0.5f * (kf.y - k1.y)};
-    }
-  }
-  const float sc = (float)C::SCALE;
-  return cpx<float>{kf.x * sc, -kf.y * sc};
-}
-
-// this tile's Kt columns, column-major into LDS (K read in 16-element row runs), four
-// elements per thread in flight at a time
-template <class C, bool EDGE>
-__device__ __forceinline__ void xcd_kt_tile(const cpx<float>* K, u32 tile, cpx<float>* lds, int tid) {
-  using X = Xcd<C>;
-  static_assert(X::IT % 4 == 0, "Kt staging in groups of four");
-#pragma nounroll
-  for (int i0 = 0; i0 < X::IT; i0 += 4) {
-    cpx<float> k[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int e = tid + (i0 + i) * X::NT;
-      k[i] = xcd_kt<C, EDGE>(K, (u32)(e >> 4), tile * 16u + (u32)(e & 15));
-    }
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int e = tid + (i0 + i) * X::NT, y = e >> 4, c = e & 15;
-      lds[c * C::NP + y + (y >> 5)] = k[i];
-    }
-  }
-}
-
-// tile task: 16 columns of the slot, in place
+// tile task: CT columns of the slot, in place
 template <class C, class TF, class TI>
-__device__ __forceinline__ void xcd_tile(const TF& tsf, const TI& tsi, const cpx<float>* K, cpx<float>* S, u32 tile,
+__device__ __forceinline__ void xcd_tile(const TF& tsf, const TI& tsi, const cpx<float>* kt, cpx<float>* S, u32 tile,
                                          cpx<float>* lds, int tid, int wave, int lane) {
   using X = Xcd<C>;
   using F0 = typename pl_first<typename C::fwd>::type;
   using FL = typename pl_last<typename C::fwd>::type;
   using IL = typename pl_last<typename C::inv>::type;
-  // 1. this tile's Kt columns, built through LDS
-  if (tile == 0 || tile == X::NB / 2) xcd_kt_tile<C, true>(K, tile, lds, tid);
-  else xcd_kt_tile<C, false>(K, tile, lds, tid);
-  __syncthreads();
-  cpx<float> kv[X::CB][X::NV];
-#pragma unroll
-  for (int cc = 0; cc < X::CB; ++cc) {
-    const cpx<float>* col = lds + (wave * X::CB + cc) * C::NP;
-    for_layout<typename X::WC, FL::R>(lane, [&](auto slot, u32 ky) { kv[cc][decltype(slot)::value] = col[ky + (ky >> 5)]; });
-  }
-  __syncthreads();
-  // 2. the tile (16 x H contiguous elements) into LDS, column-major
-  cpx<float>* T = S + tile * (u32)(16 * C::H);
+  constexpr int CT = X::CT;
+  // 1. the tile (CT x H contiguous elements) into LDS, column-major
+  cpx<float>* T = S + tile * (u32)(CT * C::H);
   {
     cpx<float> t[X::IT];
 #pragma unroll
     for (int i = 0; i < X::IT; ++i) t[i] = ld_c64_l2(T + tid + i * X::NT);
 #pragma unroll
     for (int i = 0; i < X::IT; ++i) {
-      const int e = tid + i * X::NT, y = e >> 4, c = e & 15;
+      const int e = tid + i * X::NT, y = e / CT, c = e % CT;
       lds[c * C::NP + y + (y >> 5)] = t[i];
     }
   }
   __syncthreads();
-  // 3. per wave: its columns, FFT, product, IFFT, in their own LDS regions
+  // 2. per wave: its columns, FFT, product with the Kt column (read while the FFT
+  //    runs), IFFT, in their own LDS regions
 #pragma unroll
   for (int cc = 0; cc < X::CB; ++cc) {
+    const u32 kx = tile * (u32)CT + (u32)(wave * X::CB + cc);
     cpx<float>* col = lds + (wave * X::CB + cc) * C::NP;
+    cpx<float> kv[X::NV];
+    for_layout<typename X::WC, FL::R>(lane, [&](auto slot, u32 ky) { kv[decltype(slot)::value] = kt[kx * C::H + ky]; });
     cpx<float> v[X::NV];
     for_layout<typename X::WC, F0::R>(lane, [&](auto slot, u32 n) { v[decltype(slot)::value] = col[n + (n >> 5)]; });
     tsf.template run<-1>(v, col, lane, 0);
 #pragma unroll
-    for (int k = 0; k < X::NV; ++k) v[k] = cmul(v[k], kv[cc][k]);
+    for (int k = 0; k < X::NV; ++k) v[k] = cmul(v[k], kv[k]);
     block_sync<typename X::WC>();
     tsi.template run<1>(v, col, lane, 0);
     block_sync<typename X::WC>();
     for_layout<typename X::WC, IL::R>(lane, [&](auto slot, u32 n) { col[n + (n >> 5)] = v[decltype(slot)::value]; });
   }
   __syncthreads();
-  // 4. the tile back
+  // 3. the tile back
 #pragma unroll
   for (int i = 0; i < X::IT; ++i) {
-    const int e = tid + i * X::NT, y = e >> 4, c = e & 15;
+    const int e = tid + i * X::NT, y = e / CT, c = e % CT;
     T[e] = lds[c * C::NP + y + (y >> 5)];
   }
 }
 
-// Slot 0: the images (in place), 1: the half spectrum K, 2: slot scratch (8 NSLOT
-// pair images), 3: the control block, 4: the task table (9 queue offsets, then task
-// words kind << 30 | local pair << 12 | slice).
+// Slot 0: the images (in place), 1: the half spectrum K (unused here), 2: slot
+// scratch (8 NSLOT pair images), 3: the control block, 4: the task table (9 queue
+// offsets, then task words kind << 30 | local pair << 12 | slice), 5: Kt.
 template <class C>
 __device__ __forceinline__ void conv2d_xcd_body(const KParams& prm) {
   using X = Xcd<C>;
@@ -694,10 +665,10 @@ __device__ __forceinline__ void conv2d_xcd_body(const KParams& prm) {
   __shared__ cpx<float> lds[X::LDS];
   __shared__ u32 s_word;
   float* x = bufp<float>(prm, 0);
-  const cpx<float>* K = bufp<const cpx<float>>(prm, 1);
   cpx<float>* scr = bufp<cpx<float>>(prm, 2);
   u32* ctl = bufp<u32>(prm, 3);
   const u32* table = bufp<const u32>(prm, 4);
+  const cpx<float>* kt = bufp<const cpx<float>>(prm, 5);
   const cpx<float>* twf = bufp<const cpx<float>>(prm, SLOT_TW_FWD);
   const cpx<float>* twi = bufp<const cpx<float>>(prm, SLOT_TW_INV);
 
@@ -706,11 +677,12 @@ __device__ __forceinline__ void conv2d_xcd_body(const KParams& prm) {
   const u32 xq = __builtin_amdgcn_readfirstlane(__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 7u);
   u32* err = ctl + XCTL_ERROR;
 
-  // Each task reads its lane's twiddles once, in flight with its first data loads
-  // (held across the whole launch they would cost 28 registers in every task).
+  // Twiddles come from the table (L1 / L2 hits): at four workgroups per CU the 28
+  // registers of a register-resident set would push the tasks into spills, and the
+  // table reads' latency is covered by the other workgroups of the CU.
   constexpr bool SAME = __is_same(L, LI);
-  using TSF = TwSet<typename X::WC, L, conv_twreg<typename X::WC, L>()>;
-  using TSI = TwSet<typename X::WC, LI, conv_twreg<typename X::WC, LI>()>;
+  using TSF = TwSet<typename X::WC, L, false>;
+  using TSI = TwSet<typename X::WC, LI, false>;
 
   auto run_task = [&](u32 word, u32 q) {
     // The thread index passes through an empty asm statement per task, so the
@@ -739,11 +711,11 @@ __device__ __forceinline__ void conv2d_xcd_body(const KParams& prm) {
       TSF tsf;
       tsf.load(lane, twf);
       if constexpr (SAME) {
-        xcd_tile<C>(tsf, tsf, K, S, s, lds, tid, wave, lane);
+        xcd_tile<C>(tsf, tsf, kt, S, s, lds, tid, wave, lane);
       } else {
         TSI tsi;
         tsi.load(lane, twi);
-        xcd_tile<C>(tsf, tsi, K, S, s, lds, tid, wave, lane);
+        xcd_tile<C>(tsf, tsi, kt, S, s, lds, tid, wave, lane);
       }
       signal_done(cnt + 32);
     } else {

@@ -323,6 +323,15 @@ template <class C> __device__ __forceinline__ Geo make_geo() {
 // ---------------------------------------------------------------------------
 enum { RD_INPUT = 0, RD_KERNEL = 1 };
 
+// input_signal_range: sample i of a transform is read when SIG_LO <= i < SIG_HI (zero
+// otherwise); SIG_LO > SIG_HI is a wrapped kept range [SIG_LO, N) + [0, SIG_HI), i.e. an
+// interior block [SIG_HI, SIG_LO) read as zeros (VkFFT-style zero padding).
+template <class C> constexpr bool sig_limited() { return C::SIG_LO > 0 || C::SIG_HI < C::N; }
+template <class C> __device__ __forceinline__ bool sig_kept(u32 i) {
+  if constexpr (C::SIG_LO <= C::SIG_HI) return i >= (u32)C::SIG_LO && i < (u32)C::SIG_HI;
+  else return i >= (u32)C::SIG_LO || i < (u32)C::SIG_HI;
+}
+
 template <class C, int ROLE> struct ReadOp {
   using real = typename C::real;
   using reg_t = cpx<real>;
@@ -334,8 +343,8 @@ template <class C, int ROLE> struct ReadOp {
 
   template <class S>
   __device__ __forceinline__ void read_from_buffer(const cpx<S>* buf, reg_t& dst, u32 io) const {
-    if constexpr (ROLE == RD_INPUT && (C::SIG_LO > 0 || C::SIG_HI < C::N)) {
-      if (!(fft_index >= (u32)C::SIG_LO && fft_index < (u32)C::SIG_HI)) { dst = reg_t{(real)0, (real)0}; return; }
+    if constexpr (ROLE == RD_INPUT && sig_limited<C>()) {
+      if (!sig_kept<C>(fft_index)) { dst = reg_t{(real)0, (real)0}; return; }
     }
     if constexpr (ROLE == RD_INPUT && C::R2C && !C::INV) {
       // packed real rows: buffer[io // 2][io % 2]
@@ -355,8 +364,8 @@ template <class C, int ROLE> struct ReadOp {
   // real-valued buffer: element io becomes (x, 0)
   template <class S>
   __device__ __forceinline__ void read_from_buffer(const S* buf, reg_t& dst, u32 io) const {
-    if constexpr (ROLE == RD_INPUT && (C::SIG_LO > 0 || C::SIG_HI < C::N)) {
-      if (!(fft_index >= (u32)C::SIG_LO && fft_index < (u32)C::SIG_HI)) { dst = reg_t{(real)0, (real)0}; return; }
+    if constexpr (ROLE == RD_INPUT && sig_limited<C>()) {
+      if (!sig_kept<C>(fft_index)) { dst = reg_t{(real)0, (real)0}; return; }
     }
     dst = reg_t{sc<real>(buf[io]), (real)0};
   }

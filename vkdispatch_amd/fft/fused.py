@@ -139,35 +139,40 @@ def _conv_tu(cfg: str, symbol: str, block: int, waves_per_simd: Optional[int], b
 
 class Conv2DXcd:
     """Single-launch plan (``conv2d_xcd_body``) for ``batch`` real H x W images (H == W)
-    convolved with one broadcast half spectrum.  ``waves`` waves per workgroup, 16 rows
-    per row task and 16 columns per tile task, ``wg_per_xcd`` workgroups per XCD,
-    ``nslot`` pair spectra of scratch per XCD (2 MiB each at 512 x 512)."""
+    convolved with one broadcast half spectrum: a small kernel first turns the half
+    spectrum into the column-major Kt (``conv2d_kt_body``), then one launch runs every
+    phase of every pair.  ``waves`` waves per workgroup, ``rows`` rows per row task,
+    ``ct`` columns per tile task, ``wg_per_xcd`` workgroups per XCD, ``nslot`` pair
+    spectra of scratch per XCD (2 MiB each at 512 x 512)."""
 
-    ROWS = 16
-
-    def __init__(self, batch: int, H: int, W: int, normalize: bool = True, waves: int = 16, wg_per_xcd: int = 32,
-                 nslot: int = 2, waves_per_simd: Optional[int] = None):
+    def __init__(self, batch: int, H: int, W: int, normalize: bool = True, waves: int = 4, rows: int = 8,
+                 ct: int = 8, wg_per_xcd: int = 128, nslot: int = 2, waves_per_simd: Optional[int] = 4):
         self.batch, self.H, self.W = int(batch), int(H), int(W)
         self.npairs = (self.batch + 1) // 2
         self.nslot = int(nslot)
         self.qmax = -(-self.npairs // NUM_XCD)
-        cfg, plan = _conv_cfg(batch, H, W, normalize, waves, self.ROWS,
-                              extra={"NSLOT": self.nslot, "QMAX": self.qmax})
-        self.ns = H // self.ROWS
-        self.nb = W // COL_TILE
+        self.rows, self.ct = int(rows), int(ct)
+        cfg, plan = _conv_cfg(batch, H, W, normalize, waves, self.rows,
+                              extra={"CT": self.ct, "NSLOT": self.nslot, "QMAX": self.qmax})
+        self.ns = H // self.rows
+        self.nb = W // self.ct
         grid = NUM_XCD * int(wg_per_xcd)
         self.plan = KernelPlan(**{**plan.__dict__, "NWG": grid})
-        symbol = kernel_symbol("conv2d", self.plan, "xcd")
-        src = _conv_tu(cfg, symbol, 64 * waves, waves_per_simd, "vkx::conv2d_xcd_body<Cfg>(prm);")
+        radices = list(self.plan.radices)
         args = [Arg("output", "buffer", dt.complex64, 0), Arg("kernel", "buffer", dt.complex64, 1),
                 Arg("scratch", "buffer", dt.complex64, 2), Arg("control", "buffer", dt.uint32, 3),
-                Arg("tasks", "buffer", dt.uint32, 4)]
-        radices = list(self.plan.radices)
-        self.kernel = CompiledKernel(src, self.plan, args, radices, radices[::-1],
-                                     name=f"conv2d_xcd_{batch}x{H}x{W}")
+                Arg("tasks", "buffer", dt.uint32, 4), Arg("kernel_t", "buffer", dt.complex64, 5)]
+        nk = (H // KPREP_BLOCK) * (W // KPREP_BLOCK)
+        kplan = KernelPlan(**{**plan.__dict__, "NWG": nk})
+        src_k = _conv_tu(cfg, kernel_symbol("conv2d", kplan, "xcd_kt"), 64 * waves, None,
+                         "vkx::conv2d_kt_body<Cfg>(prm);")
+        self.kt_kernel = CompiledKernel(src_k, kplan, args, None, None, name=f"conv2d_xcd_kt_{H}x{W}")
+        src = _conv_tu(cfg, kernel_symbol("conv2d", self.plan, "xcd"), 64 * waves, waves_per_simd,
+                       "vkx::conv2d_xcd_body<Cfg>(prm);")
+        self.kernel = CompiledKernel(src, self.plan, args, radices, radices[::-1], name=f"conv2d_xcd_{batch}x{H}x{W}")
         self.table = xcd_task_table(self.npairs, self.ns, self.nb, self.nslot)
         self.ctl_words = _XCTL_CNT + 64 * NUM_XCD * self.qmax
-        self._dev: Optional[Tuple[Buffer, Buffer, Buffer]] = None
+        self._dev: Optional[Tuple[Buffer, Buffer, Buffer, Buffer]] = None
         self._checked = False
         self._lock = threading.Lock()
 
@@ -179,10 +184,10 @@ class Conv2DXcd:
         return NUM_XCD * self.nslot * self.H * self.W
 
     def kernels(self) -> List[CompiledKernel]:
-        return [self.kernel]
+        return [self.kt_kernel, self.kernel]
 
-    def resources(self) -> Tuple[Buffer, Buffer, Buffer]:
-        """(scratch, control, task table); the control block starts at zero and every
+    def resources(self) -> Tuple[Buffer, Buffer, Buffer, Buffer]:
+        """(scratch, control, task table, Kt); the control block starts at zero and every
         launch leaves it at zero (error bits aside)."""
         if self._dev is None:
             with self._lock:
@@ -193,18 +198,21 @@ class Conv2DXcd:
                     ctl.write(np.zeros(self.ctl_words, np.uint32))
                     table = Buffer((self.table.size,), dt.uint32)
                     table.write(self.table)
-                    self._dev = (scratch, ctl, table)
+                    kt = Buffer((self.H, self.W), dt.complex64)
+                    self._dev = (scratch, ctl, table, kt)
         return self._dev
 
     def values(self, buffer, kernel) -> list:
         if not self._checked:
             _check_no_spill(self.kernels())
             self._checked = True
-        scratch, ctl, table = self.resources()
-        return [buffer, kernel, scratch, ctl, table]
+        scratch, ctl, table, kt = self.resources()
+        return [buffer, kernel, scratch, ctl, table, kt]
 
     def __call__(self, buffer, kernel, graph=None) -> None:
-        dispatch(self.kernel, self.values(buffer, kernel), graph)
+        vals = self.values(buffer, kernel)
+        dispatch(self.kt_kernel, vals, graph)
+        dispatch(self.kernel, vals, graph)
         rt.note_async_check(self, rt.current_stream())
 
     def _error_word(self) -> int:
@@ -314,7 +322,8 @@ def _mode(batch: int, W: int) -> str:
 
 # single-launch tilings: one 16-wave workgroup per CU, no register spills at these
 # sizes (1024 would need 128+ registers and spills: it stays on the phase kernels)
-_XCD_TILING = {256: dict(waves=16, wg_per_xcd=32, nslot=2), 512: dict(waves=16, wg_per_xcd=32, nslot=2)}
+_XCD_TILING = {256: dict(waves=4, rows=8, ct=8, wg_per_xcd=128, nslot=2),
+               512: dict(waves=4, rows=8, ct=8, wg_per_xcd=128, nslot=2)}
 # phase-kernel tilings: for 512 the row kernels run one row per wave (16 waves, 64 VGPRs,
 # no spills), so two workgroups per CU hold 8 waves per SIMD while they read HBM
 _PHASE_TILING = {512: dict(waves=16, irow_waves=8)}

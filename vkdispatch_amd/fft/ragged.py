@@ -261,8 +261,7 @@ def _launch_tasks(tasks: np.ndarray, scratch_elems_per_wg: int, grid: int, block
     ctr, nctr = counters if counters is not None else (_arena.counter_ptr(), 2)
     rc = rt.lib().vkx_ragged_exec(task_ptr, len(tasks), ctr, nctr, scr, scratch_elems_per_wg, grid, block,
                                   int(spin_limit), s)
-    if rc != 0:
-        raise RuntimeError("vkx_ragged_exec launch failed")
+    rt.check(rc, "vkx_ragged_exec")
     if task_buf is None:
         rt.synchronize(s)
         rt.free(task_ptr)

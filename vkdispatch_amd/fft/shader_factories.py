@@ -21,9 +21,20 @@ def _compute_dtype(compute_type):
     return dt.complex128 if compute_type is dt.complex128 else dt.complex64
 
 
+class ZeroBlock(tuple):
+    """An input range read as zeros, ``(l, r)`` with 0 < l < r < N: the kept samples are
+    [0, l) and [r, N) (VkFFT's interior zero padding, vd.vkfft's ``padding``).  The
+    kernel configuration encodes it as the wrapped kept range SIG_LO = r > SIG_HI = l."""
+
+
 def _resolve_signal_range(signal_range, N: int) -> Tuple[int, int]:
     if signal_range is None:
         return 0, N
+    if isinstance(signal_range, ZeroBlock):
+        lo, hi = int(signal_range[0]), int(signal_range[1])
+        if not 0 < lo < hi < N:
+            raise ValueError(f"zero block {tuple(signal_range)} must lie strictly inside [0, {N}]")
+        return hi, lo
     s, e = signal_range
     s = 0 if s is None else int(s)
     e = N if e is None else int(e)

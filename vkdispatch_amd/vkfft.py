@@ -30,7 +30,7 @@ from . import dtypes as dt
 from . import runtime as rt
 from .buffer import Buffer, RFFTBuffer
 from .fft import functions as F
-from .fft.shader_factories import cache_clear as _fft_cache_clear
+from .fft.shader_factories import ZeroBlock, cache_clear as _fft_cache_clear
 from .fft.hooks import read_op
 from .graph import CommandGraph
 from .shader_map import map as vd_map
@@ -203,8 +203,10 @@ def _product_map(inner: int, conjugate: bool):
 
 
 def _signal_range(pad, n: int):
-    """VkFFT zero range [l, r) along an axis of length n -> the kept range for
-    ``input_signal_range`` (None when nothing is zero)."""
+    """VkFFT zero range [l, r) along an axis of length n (fft_zeropad_left / right,
+    vkdispatch_native/stages/stage_fft.cpp:194-204) -> the input range the forward
+    transform reads: the kept range when the block touches an end of the axis, a
+    ZeroBlock for an interior block (None when nothing is zero)."""
     if pad is None:
         return None
     lo, hi = int(pad[0]), int(pad[1])
@@ -216,7 +218,7 @@ def _signal_range(pad, n: int):
         return (hi, n)
     if hi == n:
         return (0, lo)
-    raise NotImplementedError("zero-padding ranges must touch an end of the axis")
+    return ZeroBlock((lo, hi))
 
 
 def convolve2D(buffer: Buffer, kernel: Buffer, normalize: bool = False, conjugate_kernel: bool = False,
