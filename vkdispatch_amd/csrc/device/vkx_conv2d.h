@@ -521,6 +521,12 @@ template <class C> struct Xcd {
   static_assert(C::NP >= C::H + C::H / 32, "LDS column region");
 };
 
+// C::XNT (optional, default false): the streamed traffic (input rows, output rows, Kt)
+// uses non-temporal accesses so that it does not push the slot images out of L2
+template <class C, class = void> struct xcd_nt_of { static constexpr bool value = false; };
+template <class C> struct xcd_nt_of<C, decltype((void)C::XNT)> { static constexpr bool value = C::XNT != 0; };
+template <class C> constexpr bool xcd_nt() { return xcd_nt_of<C>::value; }
+
 template <class C> __device__ __forceinline__ u32 xcd_tile_index(u32 ky, u32 kx) {
   return (kx / (u32)C::CT) * (u32)(C::CT * C::H) + ky * (u32)C::CT + (kx % (u32)C::CT);
 }
@@ -549,8 +555,13 @@ __device__ __forceinline__ void xcd_load_rows(const float* x, u32 pair, u32 slic
     const float* pa = x + (u64)(2 * pair) * C::IMG + y * C::PITCH;
     const float* pb = pa + C::IMG;
     for_layout<typename X::WC, F0::R>(lane, [&](auto slot, u32 n) {
-      ra[j][decltype(slot)::value] = pa[n];
-      rb[j][decltype(slot)::value] = has_b ? pb[n] : 0.f;
+      if constexpr (xcd_nt<C>()) {
+        ra[j][decltype(slot)::value] = __builtin_nontemporal_load(pa + n);
+        rb[j][decltype(slot)::value] = has_b ? __builtin_nontemporal_load(pb + n) : 0.f;
+      } else {
+        ra[j][decltype(slot)::value] = pa[n];
+        rb[j][decltype(slot)::value] = has_b ? pb[n] : 0.f;
+      }
     });
   }
 }
@@ -599,8 +610,13 @@ __device__ __forceinline__ void xcd_rows_inv(const TS& ts, const cpx<float>* S, 
     float* pb = pa + C::IMG;
     for_layout<typename X::WC, IL::R>(lane, [&](auto slot, u32 n) {
       const cpx<float> z = v[j][decltype(slot)::value];
-      pa[n] = z.x;
-      if (has_b) pb[n] = z.y;
+      if constexpr (xcd_nt<C>()) {
+        __builtin_nontemporal_store(z.x, pa + n);
+        if (has_b) __builtin_nontemporal_store(z.y, pb + n);
+      } else {
+        pa[n] = z.x;
+        if (has_b) pb[n] = z.y;
+      }
     });
   }
 }
@@ -634,7 +650,14 @@ __device__ __forceinline__ void xcd_tile(const TF& tsf, const TI& tsi, const cpx
     const u32 kx = tile * (u32)CT + (u32)(wave * X::CB + cc);
     cpx<float>* col = lds + (wave * X::CB + cc) * C::NP;
     cpx<float> kv[X::NV];
-    for_layout<typename X::WC, FL::R>(lane, [&](auto slot, u32 ky) { kv[decltype(slot)::value] = kt[kx * C::H + ky]; });
+    for_layout<typename X::WC, FL::R>(lane, [&](auto slot, u32 ky) {
+      if constexpr (xcd_nt<C>()) {
+        const float* kp = reinterpret_cast<const float*>(kt + kx * C::H + ky);
+        kv[decltype(slot)::value] = cpx<float>{__builtin_nontemporal_load(kp), __builtin_nontemporal_load(kp + 1)};
+      } else {
+        kv[decltype(slot)::value] = kt[kx * C::H + ky];
+      }
+    });
     cpx<float> v[X::NV];
     for_layout<typename X::WC, F0::R>(lane, [&](auto slot, u32 n) { v[decltype(slot)::value] = col[n + (n >> 5)]; });
     tsf.template run<-1>(v, col, lane, 0);

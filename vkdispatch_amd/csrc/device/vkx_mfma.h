@@ -503,6 +503,10 @@ __device__ __forceinline__ void cross16_chunk(const Dft16Ops& o, const cpx<float
 // stage across a1 (cross16: DFT over a1), and writes X[k0][c1 + 16 d1][k2] as FP32
 // through a staging tile, 8 P contiguous bytes (P consecutive k2) per row.
 // ---------------------------------------------------------------------------
+template <class C, class = void> struct p1_ntin_of { static constexpr bool value = false; };
+template <class C> struct p1_ntin_of<C, decltype((void)C::NTIN)> { static constexpr bool value = C::NTIN != 0; };
+template <class C> constexpr bool p1_ntin() { return p1_ntin_of<C>::value; }
+
 template <class C, class SIn>
 __device__ __forceinline__ void fft3_p1_body(const KParams& prm) {
   constexpr int N = 256, PF = C::PF, P = C::P;
@@ -542,17 +546,27 @@ __device__ __forceinline__ void fft3_p1_body(const KParams& prm) {
   const Dft16Ops ops1 = dft16_ops<C::DIR>(roots, r, g, cpx<float>{t1.x * os, t1.y * os});
 
   const cpx<SIn>* rows = in + ((u64)i0 * N + (u32)a1) * N + r + 128 * h;  // row b1 at + 16 N b1
+  // C::NTIN (optional): the input is read once per transform; non-temporal loads keep it
+  // from displacing the intermediate and the output in the caches
+  auto ldin = [&](u64 off) -> cpx<SIn> {
+    if constexpr (p1_ntin<C>()) {
+      const unsigned w = __builtin_nontemporal_load(reinterpret_cast<const unsigned*>(rows + off));
+      return __builtin_bit_cast(cpx<SIn>, w);
+    } else {
+      return rows[off];
+    }
+  };
   cpx<SIn> ld[16][8];
 #pragma unroll
   for (int j = 0; j < PF; ++j)
 #pragma unroll
-    for (int i = 0; i < 8; ++i) ld[j][i] = rows[(u64)(16 * j) * N + 16 * i];
+    for (int i = 0; i < 8; ++i) ld[j][i] = ldin((u64)(16 * j) * N + 16 * i);
   cpx<float> v[16][4];
   unsigned vh[16][4];
   tiles16<NT, SIn>(ops, [&](int j, cpx<SIn> (&cur)[8]) {
     if (j + PF < 16) {
 #pragma unroll
-      for (int i = 0; i < 8; ++i) ld[j + PF][i] = rows[(u64)(16 * (j + PF)) * N + 16 * i];
+      for (int i = 0; i < 8; ++i) ld[j + PF][i] = ldin((u64)(16 * (j + PF)) * N + 16 * i);
     }
 #pragma unroll
     for (int i = 0; i < 8; ++i) cur[i] = ld[j][i];

@@ -146,14 +146,16 @@ class Conv2DXcd:
     spectra of scratch per XCD (2 MiB each at 512 x 512)."""
 
     def __init__(self, batch: int, H: int, W: int, normalize: bool = True, waves: int = 4, rows: int = 8,
-                 ct: int = 8, wg_per_xcd: int = 128, nslot: int = 2, waves_per_simd: Optional[int] = 4):
+                 ct: int = 8, wg_per_xcd: int = 128, nslot: int = 2, waves_per_simd: Optional[int] = 4,
+                 nontemporal: bool = False):
         self.batch, self.H, self.W = int(batch), int(H), int(W)
         self.npairs = (self.batch + 1) // 2
         self.nslot = int(nslot)
         self.qmax = -(-self.npairs // NUM_XCD)
         self.rows, self.ct = int(rows), int(ct)
         cfg, plan = _conv_cfg(batch, H, W, normalize, waves, self.rows,
-                              extra={"CT": self.ct, "NSLOT": self.nslot, "QMAX": self.qmax})
+                              extra={"CT": self.ct, "NSLOT": self.nslot, "QMAX": self.qmax,
+                                     "XNT": int(bool(nontemporal))})
         self.ns = H // self.rows
         self.nb = W // self.ct
         grid = NUM_XCD * int(wg_per_xcd)
@@ -308,6 +310,15 @@ class Conv2DPhases:
             dispatch(k, vals, graph)
 
 
+def _xcd_overrides() -> Dict[str, int]:
+    """VKX_XCD_TILING="key=value,...": tiling overrides of the single-launch plan."""
+    out: Dict[str, int] = {}
+    for item in filter(None, os.environ.get("VKX_XCD_TILING", "").split(",")):
+        k, _, v = item.partition("=")
+        out[k.strip()] = int(v)
+    return out
+
+
 def _mode(batch: int, W: int) -> str:
     """VKX_CONV2D_MODE ("xcd" | "phases") when set, else the measured default (the
     three phase kernels).  VKX_FUSED_CONV=0 selects the reference's three-dispatch
@@ -334,7 +345,7 @@ def fused_plan(batch: int, H: int, W: int, normalize: bool = True, mode: str = "
     if mode == "xcd":
         if W not in _XCD_TILING:
             raise ValueError(f"the single-launch 2-D convolution takes {sorted(_XCD_TILING)} images, not {W}")
-        return Conv2DXcd(batch, H, W, normalize, **_XCD_TILING[W])
+        return Conv2DXcd(batch, H, W, normalize, **{**_XCD_TILING[W], **_xcd_overrides()})
     return Conv2DPhases(batch, H, W, normalize, **_PHASE_TILING.get(W, {}))
 
 

@@ -19,6 +19,7 @@ against 768 MiB for the all-FP32 three-pass transform.
 """
 from __future__ import annotations
 
+import os
 from functools import lru_cache
 from typing import Optional, Tuple
 
@@ -70,7 +71,7 @@ def _mfma_tu(cfg: str, body: str, block: int, symbol: str, waves_per_simd: Optio
 
 def make_cube_pass(which: int, in_type, inverse: bool = False, out_scale: float = 1.0, prefetch: int = 4,
                    waves_per_simd: Optional[int] = 2, tiles: Optional[int] = None, run: int = 4,
-                   half_tiles: bool = False) -> CompiledKernel:
+                   half_tiles: bool = False, nt_input: bool = False) -> CompiledKernel:
     """One pass of the two-pass 256^3 transform (``fft3_p1_body`` / ``fft3_p2_body``):
     pass 1 reads the 16-bit volume (slot 1) and writes the fp16 intermediate (slot 0),
     pass 2 reads the intermediate (slot 1) and writes complex64 in natural order (slot 0).
@@ -97,6 +98,7 @@ def make_cube_pass(which: int, in_type, inverse: bool = False, out_scale: float 
         f"  static constexpr int BLOCK = {block}, DIR = {1 if inverse else -1}, PF = {int(prefetch)}, NT = {int(tiles)},",
         f"                       P = {int(run)};",
         f"  static constexpr bool VH = {'true' if half_tiles else 'false'};",
+        *([f"  static constexpr int NTIN = 1;"] if nt_input else []),
         f"  static constexpr double OUT_SCALE = {out_scale!r};",
         "};",
     ])
@@ -116,7 +118,7 @@ class Mixed3D:
 
     def __init__(self, shape, in_type=dt.complex_bf16, inverse: bool = False, normalize: bool = True,
                  block: int = 256, T16: Optional[int] = None, passes: int = 2, run: int = 4,
-                 half_tiles_p2: bool = False):
+                 half_tiles_p2: bool = False, nt_input: Optional[bool] = None):
         shape = tuple(int(s) for s in shape)
         if len(shape) != 3 or any(s != 256 for s in shape):
             raise ValueError("fft3_mixed handles 256 x 256 x 256 volumes")
@@ -138,7 +140,10 @@ class Mixed3D:
             # (rel-L2 2.9e-4 vs FP64 of the quantized input); half_tiles_p2=True makes them
             # fp16 too (3.6e-4): 1.2 us faster for pass 2 alone, no faster for the whole
             # transform (interleaved medians 79.2 vs 78.3 us).
-            self.passes = [make_cube_pass(1, in_type, inverse, 1.0 / 64.0, run=run, half_tiles=True),
+            if nt_input is None:
+                nt_input = os.environ.get("VKX_MIXED_NTIN", "0") == "1"
+            self.passes = [make_cube_pass(1, in_type, inverse, 1.0 / 64.0, run=run, half_tiles=True,
+                                          nt_input=bool(nt_input)),
                            make_cube_pass(2, mid, inverse, 64.0 * norm, run=run, half_tiles=half_tiles_p2)]
         else:
             nb = 256 * 256
