@@ -264,15 +264,12 @@ def bench_conv(steps, warmup, args):
 
 
 def _conv_kernel_bytes(plan, batch, n):
-    """Algorithmic bytes of each kernel of a pair-packed plan: the single launch moves
-    the step's bytes; the phase kernels each move their own inputs and outputs."""
-    from vkdispatch_amd.fft import fused
+    """Algorithmic bytes of each kernel of the pair-packed plan: each phase kernel moves
+    its own inputs and outputs (rows: images in, pair spectra and Kt out; columns: pair
+    spectra in and out, Kt in; inverse rows: pair spectra in, images out)."""
     wh = n // 2 + 1
     real_b = batch * n * n * 4
     kb = n * wh * 8
-    if isinstance(plan, fused.Conv2DXcd):
-        kt_b = n * n * 8
-        return [kb + kt_b, 2 * real_b + kt_b]
     cplx_b = (batch + 1) // 2 * n * n * 8
     kt_b = n * n * 8
     return [real_b + cplx_b + kb + kt_b, 2 * cplx_b + kt_b, cplx_b + real_b]

@@ -148,11 +148,6 @@ int vkx_roots(uint64_t dst, int n, int double_precision, uint64_t stream);
 int vkx_batch_sum(uint64_t dst, uint64_t src, int batch, long long count, int double_precision, uint64_t stream);
 /* Bluestein chirp w[m] = exp(sign * i pi m^2 / n), m in [0, count); m^2 mod 2n in exact integers. */
 int vkx_chirp(uint64_t dst, int n, int count, int sign, int double_precision, uint64_t stream);
-/* out[w] = XCD (XCC_ID register) that workgroup w of an nwg-workgroup launch ran on
- * (synchronous, current device).  No counterpart in the reference: the fused 2-D
- * convolution keeps each image pair on one XCD and checks the workgroup-to-XCD
- * placement with this probe before it is enabled. */
-int vkx_xcc_map(int* out, int nwg);
 
 /* ---- ragged batches ------------------------------------------------------------ */
 /* No counterpart in the reference (its FFTs are one kernel per shape, one length
@@ -161,12 +156,12 @@ int vkx_xcc_map(int* out, int nwg);
  * in vkdispatch_amd/fft/ragged.py, RTASK_DTYPE) in queue order, where an item only
  * waits for items before it; `counters` holds `ncounters` >= 2 device ints, all
  * zeroed by the call (queue position, abort flag, per-transform done counters);
- * `scratch` holds `scratch_per_wg` complex64 elements per workgroup of `grid`;
+ * `workspace` holds `scratch_per_wg` complex64 elements per workgroup of `grid`;
  * `block` in {64, 128, ..., 512}.  A waiting item polls its counter at most
  * `spin_limit` times (<= 0: 2^20), then sets the abort flag (counters[1] = 1) and
  * every waiting workgroup stops waiting, so the launch always ends; a result with
  * the flag set is invalid. */
-int vkx_ragged_exec(uint64_t tasks, int ntasks, uint64_t counters, int ncounters, uint64_t scratch,
+int vkx_ragged_exec(uint64_t tasks, int ntasks, uint64_t counters, int ncounters, uint64_t workspace,
                     long long scratch_per_wg, int grid, int block, int spin_limit, uint64_t stream);
 /* Bluestein kernel b[j] = exp(+i pi j^2 / n) for |j| < n (indices mod m), 0 elsewhere,
  * complex64, phases in FP64 with j^2 reduced mod 2n in integers; m >= 2n-1. */

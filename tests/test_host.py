@@ -246,9 +246,7 @@ def test_pair_conv_kernels_do_not_spill():
     spilling kernels at launch, so the defaults must not spill)."""
     from vkdispatch_amd.fft import fused
     for batch, n, norm in fused._STANDARD_FUSED:
-        for mode in ("phases", "xcd"):
-            if mode == "xcd" and n not in fused._XCD_TILING:
-                continue
+        for mode in ("phases",):
             for k in fused.fused_plan(batch, n, n, norm, mode).kernels():
                 info = k.code_object_info()
                 assert info["private_bytes"] == 0, (k.name, info)

@@ -54,7 +54,6 @@ def test_real_src_is_what_the_call_launches(name, shape, launched):
 def test_convolve2DR_src_is_what_the_call_launches(shape, path, launched, monkeypatch):
     # the pair plans would allocate their scratch at the first call: record instead
     monkeypatch.setattr(fused.Conv2DPhases, "values", lambda self, b, k: [b, k])
-    monkeypatch.setattr(fused.Conv2DXcd, "values", lambda self, b, k: [b, k])
     buf = _fake_rfft(shape)
     ker = _fake_rfft(shape[-2:])
     vd.fft.convolve2DR(buf, ker, kernel_inner_only=True)

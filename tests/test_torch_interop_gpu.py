@@ -66,7 +66,7 @@ def test_rfftbuffer_alias_of_a_torch_tensor():
     assert_close(spec, np.fft.rfft2(x.astype(np.float64)))
 
 
-@pytest.mark.parametrize("mode", ["phases", "xcd"])
+@pytest.mark.parametrize("mode", ["phases"])
 def test_convolve2DR_inside_torch_cuda_graph(mode):
     """The captured graph replays the convolution on whatever the input tensor holds at
     replay time; eager vd work runs between replays."""

@@ -104,12 +104,6 @@ __global__ void batch_sum_kernel(T* __restrict__ dst, const T* __restrict__ src,
   dst[i] = acc;
 }
 
-// out[wg] = XCC_ID hardware register of workgroup wg (which XCD it runs on)
-__global__ void xcc_probe_kernel(int* out, int nwg) {
-  if (threadIdx.x == 0 && (int)blockIdx.x < nwg)
-    out[blockIdx.x] = (int)(__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 15u);
-}
-
 // w[m] = exp(sign * i * pi * m^2 / n) with m^2 reduced mod 2n in integers.
 template <typename T>
 __global__ void chirp_kernel(T* out, int n, int count, int sign) {
@@ -463,19 +457,6 @@ int vkx_batch_sum(uint64_t dst, uint64_t src, int batch, long long count, int do
     hipLaunchKernelGGL(batch_sum_kernel<float>, dim3((unsigned)blocks), dim3(threads), 0, S(stream),
                        reinterpret_cast<float*>(dst), reinterpret_cast<const float*>(src), batch, count);
   VKX_HIP(hipGetLastError());
-  return 0;
-}
-
-int vkx_xcc_map(int* out, int nwg) {
-  if (nwg < 1 || nwg > (1 << 16)) return fail("vkx_xcc_map", "bad workgroup count");
-  int* d = nullptr;
-  VKX_HIP(hipMalloc(&d, sizeof(int) * (size_t)nwg));
-  hipLaunchKernelGGL(xcc_probe_kernel, dim3((unsigned)nwg), dim3(64), 0, 0, d, nwg);
-  hipError_t e = hipGetLastError();
-  if (e == hipSuccess) e = hipMemcpy(out, d, sizeof(int) * (size_t)nwg, hipMemcpyDeviceToHost);
-  hipError_t f = hipFree(d);
-  VKX_HIP(e);
-  VKX_HIP(f);
   return 0;
 }
 

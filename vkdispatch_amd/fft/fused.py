@@ -8,20 +8,15 @@ to the reference's three-dispatch composition (rfft -> convolve(axis ndim-2) ->
 irfft, vkdispatch/fft/functions.py:614-646).  Both paths compute
 irfft2(rfft2(x) . conj(K)) per image (vkdispatch/fft/shader_factories.py:136-145).
 
-Two pair-packed forms: ``Conv2DXcd`` (one launch; per-XCD task queues keep each
-pair's transposes inside one XCD's L2) and ``Conv2DPhases`` (three kernels in
-stream order through a scratch image per pair).  VKX_CONV2D_MODE=xcd|phases
-overrides the choice; VKX_FUSED_CONV=0 forces the three-dispatch composition.
+The pair-packed form is ``Conv2DPhases``: three kernels in stream order through a
+scratch image per pair.  VKX_FUSED_CONV=0 forces the three-dispatch composition.
 """
 from __future__ import annotations
 
-import ctypes
 import os
 import threading
 from functools import lru_cache
 from typing import Dict, List, Optional, Sequence, Tuple
-
-import numpy as np
 
 from .. import dtypes as dt
 from .. import runtime as rt
@@ -32,58 +27,11 @@ from .kernels import CompiledKernel, check_no_spill, kernel_symbol, source_symbo
 from .planner import KernelPlan
 from .shader_factories import make_convolution_shader, make_fft_shader, register_plan_cache
 
-NUM_XCD = 8
 _FUSED_RADICES = {256: (4, 4, 4, 4), 512: (8, 8, 8), 1024: (16, 16, 4)}
-_XT_A, _XT_B, _XT_CA, _XT_C = 0, 1, 2, 3
-_XCTL_ERROR = 288
-_XCTL_CNT = 512
 
 
 def _fused_enabled() -> bool:
     return os.environ.get("VKX_FUSED_CONV", "1") != "0"
-
-
-_placement: Dict[Tuple[int, int], bool] = {}
-_placement_lock = threading.Lock()
-
-
-def xcd_placement_ok(nwg: int) -> bool:
-    """True when workgroup w of an ``nwg``-workgroup launch of a small probe kernel runs
-    on XCD w % 8 (XCC_ID register; diagnostic only: the single-pass kernel picks its
-    queue from the register and does not depend on the placement)."""
-    rt.ensure_device()
-    key = (0, int(nwg))
-    with _placement_lock:
-        if key not in _placement:
-            out = (ctypes.c_int * nwg)()
-            rt.check(rt.lib().vkx_xcc_map(out, nwg), "vkx_xcc_map")
-            _placement[key] = all(out[w] == w % NUM_XCD for w in range(nwg))
-        return _placement[key]
-
-
-def xcd_task_table(npairs: int, ns: int, nb: int, nslot: int) -> np.ndarray:
-    """Per-XCD task queues of the single-launch kernel: words 0..8 are queue offsets,
-    then task words (kind << 30 | local pair << 12 | slice).  Queue q holds pairs
-    q, q + 8, ... (local index i); pair i uses slot i % nslot.  Order: the row tasks
-    of the first nslot pairs, then per group of nslot pairs their tile tasks followed
-    by their row-inverse tasks, which also bring in pair i + nslot (CA) when it
-    exists (C otherwise)."""
-    queues: List[List[int]] = []
-    for q in range(NUM_XCD):
-        n = len(range(q, npairs, NUM_XCD))
-        words: List[int] = []
-        for i in range(min(nslot, n)):
-            words += [(_XT_A << 30) | (i << 12) | s for s in range(ns)]
-        for g in range(0, n, nslot):
-            grp = range(g, min(g + nslot, n))
-            for i in grp:
-                words += [(_XT_B << 30) | (i << 12) | c for c in range(nb)]
-            for i in grp:
-                kind = _XT_CA if i + nslot < n else _XT_C
-                words += [(kind << 30) | (i << 12) | s for s in range(ns)]
-        queues.append(words)
-    offs = np.cumsum([0] + [len(w) for w in queues]).astype(np.uint32)
-    return np.concatenate([offs, np.array([w for ws in queues for w in ws], dtype=np.uint32)])
 
 
 COL_TILE = 16      # columns per column task (128-byte tile rows)
@@ -93,7 +41,7 @@ KPREP_BLOCK = 64   # kernel-prep task: 64 x 64 elements of Kt
 def _conv_cfg(batch: int, H: int, W: int, normalize: bool, waves: int, rows: int, layout: int = 0,
               cols_per_wave: int = 1, pairs_per_task: int = 1, extra: Optional[Dict[str, int]] = None
               ) -> Tuple[str, KernelPlan]:
-    """``struct Cfg`` of the cooperative / phase kernels and the matching KernelPlan
+    """``struct Cfg`` of the phase kernels and the matching KernelPlan
     (radices, twiddle tables)."""
     if H != W or W not in _FUSED_RADICES:
         raise ValueError("the pair-packed 2-D convolution takes square 256, 512 or 1024 images")
@@ -135,113 +83,6 @@ def _conv_tu(cfg: str, symbol: str, block: int, waves_per_simd: Optional[int], b
     return "\n".join(['#include "vkx_conv2d.h"', "", cfg, "",
                       f'extern "C" __global__ void __launch_bounds__({block}){occ} {symbol}(vkx::KParams prm) {{',
                       f"  {body}", "}", ""])
-
-
-class Conv2DXcd:
-    """Single-launch plan (``conv2d_xcd_body``) for ``batch`` real H x W images (H == W)
-    convolved with one broadcast half spectrum: a small kernel first turns the half
-    spectrum into the column-major Kt (``conv2d_kt_body``), then one launch runs every
-    phase of every pair.  ``waves`` waves per workgroup, ``rows`` rows per row task,
-    ``ct`` columns per tile task, ``wg_per_xcd`` workgroups per XCD, ``nslot`` pair
-    spectra of scratch per XCD (2 MiB each at 512 x 512)."""
-
-    def __init__(self, batch: int, H: int, W: int, normalize: bool = True, waves: int = 4, rows: int = 8,
-                 ct: int = 8, wg_per_xcd: int = 128, nslot: int = 2, waves_per_simd: Optional[int] = 4,
-                 nontemporal: bool = False):
-        self.batch, self.H, self.W = int(batch), int(H), int(W)
-        self.npairs = (self.batch + 1) // 2
-        self.nslot = int(nslot)
-        self.qmax = -(-self.npairs // NUM_XCD)
-        self.rows, self.ct = int(rows), int(ct)
-        cfg, plan = _conv_cfg(batch, H, W, normalize, waves, self.rows,
-                              extra={"CT": self.ct, "NSLOT": self.nslot, "QMAX": self.qmax,
-                                     "XNT": int(bool(nontemporal))})
-        self.ns = H // self.rows
-        self.nb = W // self.ct
-        grid = NUM_XCD * int(wg_per_xcd)
-        self.plan = KernelPlan(**{**plan.__dict__, "NWG": grid})
-        radices = list(self.plan.radices)
-        args = [Arg("output", "buffer", dt.complex64, 0), Arg("kernel", "buffer", dt.complex64, 1),
-                Arg("scratch", "buffer", dt.complex64, 2), Arg("control", "buffer", dt.uint32, 3),
-                Arg("tasks", "buffer", dt.uint32, 4), Arg("kernel_t", "buffer", dt.complex64, 5)]
-        nk = (H // KPREP_BLOCK) * (W // KPREP_BLOCK)
-        kplan = KernelPlan(**{**plan.__dict__, "NWG": nk})
-        src_k = _conv_tu(cfg, kernel_symbol("conv2d", kplan, "xcd_kt"), 64 * waves, None,
-                         "vkx::conv2d_kt_body<Cfg>(prm);")
-        self.kt_kernel = CompiledKernel(src_k, kplan, args, None, None, name=f"conv2d_xcd_kt_{H}x{W}")
-        src = _conv_tu(cfg, kernel_symbol("conv2d", self.plan, "xcd"), 64 * waves, waves_per_simd,
-                       "vkx::conv2d_xcd_body<Cfg>(prm);")
-        self.kernel = CompiledKernel(src, self.plan, args, radices, radices[::-1], name=f"conv2d_xcd_{batch}x{H}x{W}")
-        self.table = xcd_task_table(self.npairs, self.ns, self.nb, self.nslot)
-        self.ctl_words = _XCTL_CNT + 64 * NUM_XCD * self.qmax
-        self._dev: Optional[Tuple[Buffer, Buffer, Buffer, Buffer]] = None
-        self._checked = False
-        self._lock = threading.Lock()
-
-    @property
-    def grid(self) -> int:
-        return self.plan.NWG
-
-    def scratch_elems(self) -> int:
-        return NUM_XCD * self.nslot * self.H * self.W
-
-    def kernels(self) -> List[CompiledKernel]:
-        return [self.kt_kernel, self.kernel]
-
-    def resources(self) -> Tuple[Buffer, Buffer, Buffer, Buffer]:
-        """(scratch, control, task table, Kt); the control block starts at zero and every
-        launch leaves it at zero (error bits aside)."""
-        if self._dev is None:
-            with self._lock:
-                if self._dev is None:
-                    rt.require_not_capturing("the first call of a 2-D convolution plan")
-                    scratch = Buffer((NUM_XCD * self.nslot * self.H, self.W), dt.complex64)
-                    ctl = Buffer((self.ctl_words,), dt.uint32)
-                    ctl.write(np.zeros(self.ctl_words, np.uint32))
-                    table = Buffer((self.table.size,), dt.uint32)
-                    table.write(self.table)
-                    kt = Buffer((self.H, self.W), dt.complex64)
-                    self._dev = (scratch, ctl, table, kt)
-        return self._dev
-
-    def values(self, buffer, kernel) -> list:
-        if not self._checked:
-            _check_no_spill(self.kernels())
-            self._checked = True
-        scratch, ctl, table, kt = self.resources()
-        return [buffer, kernel, scratch, ctl, table, kt]
-
-    def __call__(self, buffer, kernel, graph=None) -> None:
-        vals = self.values(buffer, kernel)
-        dispatch(self.kt_kernel, vals, graph)
-        dispatch(self.kernel, vals, graph)
-        rt.note_async_check(self, rt.current_stream())
-
-    def _error_word(self) -> int:
-        ctl = self.resources()[1]
-        w = np.zeros(1, np.uint32)
-        rt.dtoh(w.ctypes.data, ctl.ptr + 4 * _XCTL_ERROR, 4)
-        return int(w[0])
-
-    def error_bits(self) -> int:
-        """Error word of the control block (bit 0: a dependency wait expired, the result
-        is invalid; bit 2: a queue had no workgroup on its XCD and the last workgroup ran
-        it, which is correct but slow).  Synchronises the stream."""
-        rt.check(rt.lib().vkx_stream_synchronize(rt.current_stream()), "vkx_stream_synchronize")
-        rt.forget_async_check(self)
-        return self._error_word()
-
-    def async_error(self):
-        """Message for synchronize() when a launch's dependency wait expired."""
-        if self._error_word() & 1:
-            return (f"2-D convolution of {self.batch} images {self.H}x{self.W}: a dependency wait between "
-                    "workgroups expired; the result is invalid")
-        return None
-
-    def clear_error(self) -> None:
-        ctl = self.resources()[1]
-        rt.synchronize()
-        ctl.write(np.zeros(self.ctl_words, np.uint32))
 
 
 class Conv2DPhases:
@@ -310,31 +151,15 @@ class Conv2DPhases:
             dispatch(k, vals, graph)
 
 
-def _xcd_overrides() -> Dict[str, int]:
-    """VKX_XCD_TILING="key=value,...": tiling overrides of the single-launch plan."""
-    out: Dict[str, int] = {}
-    for item in filter(None, os.environ.get("VKX_XCD_TILING", "").split(",")):
-        k, _, v = item.partition("=")
-        out[k.strip()] = int(v)
-    return out
-
-
 def _mode(batch: int, W: int) -> str:
-    """VKX_CONV2D_MODE ("xcd" | "phases") when set, else the measured default (the
-    three phase kernels).  VKX_FUSED_CONV=0 selects the reference's three-dispatch
-    composition instead (see fused_eligible)."""
-    m = os.environ.get("VKX_CONV2D_MODE")
-    if m:
-        if m not in ("xcd", "phases"):
-            raise ValueError("VKX_CONV2D_MODE is 'xcd' or 'phases'")
-        return m
-    return "phases"
+    """VKX_CONV2D_MODE ("phases", the only pair-packed form).  VKX_FUSED_CONV=0 selects
+    the reference's three-dispatch composition instead (see fused_eligible)."""
+    m = os.environ.get("VKX_CONV2D_MODE", "phases")
+    if m != "phases":
+        raise ValueError("VKX_CONV2D_MODE is 'phases' (the single-launch form was measured slower and removed)")
+    return m
 
 
-# single-launch tilings: one 16-wave workgroup per CU, no register spills at these
-# sizes (1024 would need 128+ registers and spills: it stays on the phase kernels)
-_XCD_TILING = {256: dict(waves=4, rows=8, ct=8, wg_per_xcd=128, nslot=2),
-               512: dict(waves=4, rows=8, ct=8, wg_per_xcd=128, nslot=2)}
 # phase-kernel tilings: for 512 the row kernels run one row per wave (16 waves, 64 VGPRs,
 # no spills), so two workgroups per CU hold 8 waves per SIMD while they read HBM
 _PHASE_TILING = {512: dict(waves=16, irow_waves=8)}
@@ -342,10 +167,6 @@ _PHASE_TILING = {512: dict(waves=16, irow_waves=8)}
 
 @lru_cache(maxsize=32)
 def fused_plan(batch: int, H: int, W: int, normalize: bool = True, mode: str = "phases"):
-    if mode == "xcd":
-        if W not in _XCD_TILING:
-            raise ValueError(f"the single-launch 2-D convolution takes {sorted(_XCD_TILING)} images, not {W}")
-        return Conv2DXcd(batch, H, W, normalize, **{**_XCD_TILING[W], **_xcd_overrides()})
     return Conv2DPhases(batch, H, W, normalize, **_PHASE_TILING.get(W, {}))
 
 
@@ -375,8 +196,8 @@ def plan_for_shape(real_shape, kernel_map=None, transposed_kernel=False, kernel_
 
 def fused_eligible(buffer, kernel, kernel_map=None, buffer_shape=None, transposed_kernel=False,
                    kernel_inner_only=False, normalize=True, compute_type=None, **_):
-    """The pair-packed plan (Conv2DXcd or Conv2DPhases) for this convolve2DR call, or
-    None when the call needs the three-dispatch composition."""
+    """The pair-packed plan (Conv2DPhases) for this convolve2DR call, or None when the
+    call needs the three-dispatch composition."""
     if getattr(buffer, "var_type", None) is not dt.complex64 or getattr(kernel, "var_type", None) is not dt.complex64:
         return None
     rs = tuple(getattr(buffer, "real_shape", ()))
@@ -423,10 +244,7 @@ _STANDARD_FUSED = ((64, 512, True), (8, 512, True), (16, 512, True), (6, 512, Tr
 def standard_kernels() -> List:
     ks = list(composed_kernels((64, 512, 512))) + list(composed_kernels((16, 512, 512)))
     for batch, n, norm in _STANDARD_FUSED:
-        for mode in ("phases", "xcd"):
-            if mode == "xcd" and n not in _XCD_TILING:
-                continue
-            ks += fused_plan(batch, n, n, norm, mode).kernels()
+        ks += fused_plan(batch, n, n, norm).kernels()
     return ks
 
 

@@ -97,7 +97,6 @@ _SIGS = {
     "vkx_roots": ([_U64, ctypes.c_int, ctypes.c_int, _U64], ctypes.c_int),
     "vkx_batch_sum": ([_U64, _U64, ctypes.c_int, ctypes.c_longlong, ctypes.c_int, _U64], ctypes.c_int),
     "vkx_chirp": ([_U64, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _U64], ctypes.c_int),
-    "vkx_xcc_map": ([ctypes.POINTER(ctypes.c_int), ctypes.c_int], ctypes.c_int),
     "vkx_ragged_exec": ([_U64, ctypes.c_int, _U64, ctypes.c_int, _U64, ctypes.c_longlong, ctypes.c_int, ctypes.c_int,
                          ctypes.c_int, _U64],
                         ctypes.c_int),
