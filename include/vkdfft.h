@@ -93,6 +93,11 @@ int vkx_event_elapsed_ms(float* ms, uint64_t start, uint64_t stop);
  * (may be NULL when empty; free with vkx_free_host). */
 int vkx_compile(const char* source, const char* name, const char* const* options, int num_options,
                 void** code, size_t* code_size, char** log);
+/* release of the HIP installation whose hiprtc vkx_compile runs (hipRuntimeGetVersion;
+ * part of the on-disk code-object cache key, so code built by one compiler release is
+ * never loaded as another's); no counterpart in the reference, which keeps no on-disk
+ * code-object cache */
+int vkx_compiler_version(int* version);
 void vkx_free_host(void* p);
 /* pipeline creation half of stage_compute_plan_create (stage_compute.cpp:84-250) */
 int vkx_module_load(uint64_t* module, const void* code, size_t code_size);

@@ -36,60 +36,53 @@ def _env(name, value):
             os.environ[name] = old
 
 
-def _run(x, K, kernel_inner_only=True, normalize=True, mode="phases"):
+def _run(x, K, kernel_inner_only=True, normalize=True):
     buf = vd.asrfftbuffer(x)
     kb = vd.RFFTBuffer(x.shape[-2:])
     kb.write_fourier(K)
-    with _env("VKX_CONV2D_MODE", mode):
-        plan = fused.fused_eligible(buf, kb, kernel_inner_only=kernel_inner_only, normalize=normalize)
-        assert plan is not None, "expected the pair-packed path"
-        assert isinstance(plan, fused.Conv2DPhases)
-        vd.fft.convolve2DR(buf, kb, kernel_inner_only=kernel_inner_only, normalize=normalize)
+    plan = fused.fused_eligible(buf, kb, kernel_inner_only=kernel_inner_only, normalize=normalize)
+    assert plan is not None, "expected the pair-packed path"
+    assert isinstance(plan, fused.Conv2DPhases)
+    vd.fft.convolve2DR(buf, kb, kernel_inner_only=kernel_inner_only, normalize=normalize)
     vd.synchronize()
     return buf.read_real(0)
 
 
-MODES = ["phases"]
-
-
-@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("batch,n", [(8, 512), (5, 256), (2, 1024), (1, 512), (17, 512), (40, 256)])
-def test_fused_matches_oracle(batch, n, mode):
-    print(f"pair-packed convolve2DR {mode} batch={batch} n={n}", flush=True)
+def test_fused_matches_oracle(batch, n):
+    print(f"pair-packed convolve2DR batch={batch} n={n}", flush=True)
     rng = np.random.default_rng(batch * 1000 + n)
     x = rng.uniform(-1, 1, (batch, n, n)).astype(np.float32)
     k = _gauss_kernel(rng, n)
     K = np.fft.rfft2(k.astype(np.float64)).astype(np.complex64)
     if batch == 1:
         x = x[0]
-    got = _run(x, K, mode=mode)
+    got = _run(x, K)
     ref = O.convolve2DR(x, K, kernel_inner_only=True)
     e = assert_close(got, ref, rel=1e-5)
     print(f"  rel-L2 {e:.3e}", flush=True)
 
 
-@pytest.mark.parametrize("mode", MODES)
-def test_fused_non_hermitian_half_spectrum(mode):
+def test_fused_non_hermitian_half_spectrum():
     """A half spectrum that is not Hermitian on the DC and Nyquist columns: the
     pair-packed path must reproduce what rfft -> convolve -> irfft does with it."""
-    print(f"pair-packed convolve2DR {mode} non-Hermitian K", flush=True)
+    print("pair-packed convolve2DR non-Hermitian K", flush=True)
     rng = np.random.default_rng(7)
     n = 256
     x = rng.uniform(-1, 1, (4, n, n)).astype(np.float32)
     K = (rng.standard_normal((n, n // 2 + 1)) + 1j * rng.standard_normal((n, n // 2 + 1))).astype(np.complex64)
-    got = _run(x, K, mode=mode)
+    got = _run(x, K)
     ref = O.convolve2DR(x, K, kernel_inner_only=True)
     assert_close(got, ref, rel=1e-5)
 
 
-@pytest.mark.parametrize("mode", MODES)
-def test_fused_unnormalized(mode):
-    print(f"pair-packed convolve2DR {mode} normalize=False", flush=True)
+def test_fused_unnormalized():
+    print("pair-packed convolve2DR normalize=False", flush=True)
     rng = np.random.default_rng(8)
     n = 256
     x = rng.uniform(-1, 1, (3, n, n)).astype(np.float32)
     K = np.fft.rfft2(_gauss_kernel(rng, n).astype(np.float64)).astype(np.complex64)
-    got = _run(x, K, normalize=False, mode=mode)
+    got = _run(x, K, normalize=False)
     ref = O.convolve2DR(x, K, normalize=False, kernel_inner_only=True)
     assert_close(got, ref, rel=1e-5)
 
@@ -99,7 +92,7 @@ def test_fused_agrees_with_three_dispatch_path():
     rng = np.random.default_rng(9)
     x = rng.uniform(-1, 1, (16, 512, 512)).astype(np.float32)
     K = np.fft.rfft2(_gauss_kernel(rng, 512).astype(np.float64)).astype(np.complex64)
-    got = _run(x, K, mode="phases")
+    got = _run(x, K)
     with _env("VKX_FUSED_CONV", "0"):
         buf = vd.asrfftbuffer(x)
         kb = vd.RFFTBuffer((512, 512))
@@ -110,28 +103,25 @@ def test_fused_agrees_with_three_dispatch_path():
     assert O.rel_l2(got, three) <= 2e-6
 
 
-@pytest.mark.parametrize("mode", MODES)
-def test_fused_repeated_launches_and_graph_replays(mode):
+def test_fused_repeated_launches_and_graph_replays():
     """Back-to-back launches and graph replays give the same result as the first
-    launch (the cooperative kernel's control block resets itself every launch)."""
-    print(f"pair-packed convolve2DR {mode} repeated launches / graph replays", flush=True)
+    launch (the plan's scratch carries nothing from one call to the next)."""
+    print("pair-packed convolve2DR repeated launches / graph replays", flush=True)
     rng = np.random.default_rng(10)
     n = 512
     x = rng.uniform(-1, 1, (6, n, n)).astype(np.float32)
     K = np.fft.rfft2(_gauss_kernel(rng, n).astype(np.float64)).astype(np.complex64)
     ref = O.convolve2DR(x, K, kernel_inner_only=True)
-    first = _run(x, K, mode=mode)
+    first = _run(x, K)
     assert_close(first, ref, rel=1e-5)
     for _ in range(3):
-        assert np.array_equal(_run(x, K, mode=mode), first)
+        assert np.array_equal(_run(x, K), first)
     # graph: the recorded dispatches replayed on fresh copies of the input
     buf = vd.asrfftbuffer(x)
     kb = vd.RFFTBuffer((n, n))
     kb.write_fourier(K)
     g = vd.CommandGraph()
-    with _env("VKX_CONV2D_MODE", mode):
-        vd.fft.convolve2DR(buf, kb, kernel_inner_only=True, graph=g)
-        plan = fused.fused_eligible(buf, kb, kernel_inner_only=True)
+    vd.fft.convolve2DR(buf, kb, kernel_inner_only=True, graph=g)
     for _ in range(3):
         buf.write_real(x)
         g.submit()
@@ -146,18 +136,17 @@ def test_fused_not_used_for_per_image_kernels():
     assert fused.fused_eligible(buf, kb, kernel_inner_only=False) is None
 
 
-@pytest.mark.parametrize("mode", MODES)
-def test_headline_workload_every_image(mode):
+def test_headline_workload_every_image():
     """The benchmark's workload exactly: 64 x 512^2 U(-1, 1) images from seed 0, one
     512^2 kernel (Gaussian sigma 4 + 1% noise) as a pre-transformed half spectrum,
     default convolve2DR with the broadcast kernel; all 64 images against the FP64
     oracle (reference composition: vkdispatch/fft/functions.py:614-646,
     tests/test_conv.py:132-157)."""
-    print(f"headline 64 x 512^2 convolve2DR, mode {mode}, all images", flush=True)
+    print("headline 64 x 512^2 convolve2DR, all images", flush=True)
     import bench
     x, k = bench.conv_inputs(64, 512, 0)
     K = np.fft.rfft2(k.astype(np.float64)).astype(np.complex64)
-    got = _run(x, K, mode=mode)
+    got = _run(x, K)
     ref = O.convolve2DR(x, K, kernel_inner_only=True)
     worst = max(O.rel_l2(got[b], ref[b]) for b in range(64))
     print(f"  worst per-image rel-L2 {worst:.3e}, whole batch {O.rel_l2(got, ref):.3e}", flush=True)

@@ -1,7 +1,6 @@
 """CPU-only tests: the C-ABI library, the planner, precision rules, hook tracing and
 kernel compilation (hiprtc cross-compiles gfx950 without a GPU).  No test here
 launches device work."""
-import ctypes
 import os
 import re
 from types import SimpleNamespace
@@ -246,10 +245,9 @@ def test_pair_conv_kernels_do_not_spill():
     spilling kernels at launch, so the defaults must not spill)."""
     from vkdispatch_amd.fft import fused
     for batch, n, norm in fused._STANDARD_FUSED:
-        for mode in ("phases",):
-            for k in fused.fused_plan(batch, n, n, norm, mode).kernels():
-                info = k.code_object_info()
-                assert info["private_bytes"] == 0, (k.name, info)
+        for k in fused.fused_plan(batch, n, n, norm).kernels():
+            info = k.code_object_info()
+            assert info["private_bytes"] == 0, (k.name, info)
 
 
 def test_map_resolves_postponed_annotations():

@@ -4,6 +4,7 @@
 the CommandGraph's ownership of the buffers it recorded."""
 import ctypes
 import gc
+import os
 import weakref
 
 import pytest
@@ -136,3 +137,18 @@ def test_cuda_graph_capture_scope_routes_the_stream_and_restores_it():
             assert vd.get_cuda_capture() is None and rt._state.stream_override is None
         assert vd.get_cuda_capture() is cap and rt._state.stream_override == 0x1234
     assert vd.get_cuda_capture() is None and rt._state.stream_override == before
+
+
+def test_one_hip_runtime_per_process():
+    """Loading libvkdfft.so first must not leave torch with a second HIP runtime: torch's
+    wheel carries its own libamdhip64 under the same SONAME, and a copy loaded after the
+    system one finds no GPU.  rt.lib() imports torch first, so one copy serves both."""
+    import subprocess
+    import sys
+    code = ("import vkdispatch_amd.runtime as rt; rt.lib(); import torch\n"
+            "libs = {l.split()[-1] for l in open('/proc/self/maps') if 'libamdhip64' in l}\n"
+            "print(len(libs))\n")
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300,
+                         cwd=os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.strip().splitlines()[-1] == "1"

@@ -4,9 +4,6 @@ through the vd.cuda_graph_capture scope (vkdispatch/execution_pipeline/
 cuda_graph_capture.py:22-52; the reference's examples/pytorch_cuda_graph_cuda_python.py),
 replayed several times with eager work between the replays; every replay is checked
 against the FP64 oracle."""
-import contextlib
-import os
-
 import numpy as np
 import pytest
 
@@ -17,19 +14,6 @@ from _gpu_common import assert_close
 
 pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
-
-
-@contextlib.contextmanager
-def _env(name, value):
-    old = os.environ.get(name)
-    os.environ[name] = value
-    try:
-        yield
-    finally:
-        if old is None:
-            del os.environ[name]
-        else:
-            os.environ[name] = old
 
 
 def test_from_cuda_array_and_to_torch_alias_the_same_memory():
@@ -66,11 +50,10 @@ def test_rfftbuffer_alias_of_a_torch_tensor():
     assert_close(spec, np.fft.rfft2(x.astype(np.float64)))
 
 
-@pytest.mark.parametrize("mode", ["phases"])
-def test_convolve2DR_inside_torch_cuda_graph(mode):
+def test_convolve2DR_inside_torch_cuda_graph():
     """The captured graph replays the convolution on whatever the input tensor holds at
     replay time; eager vd work runs between replays."""
-    print(f"interop: convolve2DR ({mode}) captured in torch.cuda.graph", flush=True)
+    print("interop: convolve2DR captured in torch.cuda.graph", flush=True)
     rng = np.random.default_rng(2)
     B, n = 16, 512
     xs = [rng.uniform(-1, 1, (B, n, n)).astype(np.float32) for _ in range(3)]
@@ -80,20 +63,19 @@ def test_convolve2DR_inside_torch_cuda_graph(mode):
     k_t = torch.from_numpy(K).to("cuda")
     buf = vd.rfftbuffer_from_cuda_array(x_t, (B, n, n))
     kb = vd.rfftbuffer_from_cuda_array(k_t, (n, n))
-    with _env("VKX_CONV2D_MODE", mode):
-        # warm-up outside the capture: the plan allocates its scratch on first use
-        x_t[..., :n] = torch.from_numpy(xs[0]).to("cuda")
-        torch.cuda.synchronize()
-        vd.fft.convolve2DR(buf, kb, kernel_inner_only=True)
-        vd.synchronize()
-        assert_close(x_t[..., :n].cpu().numpy(), O.convolve2DR(xs[0], K, kernel_inner_only=True), rel=1e-5)
-        cmd = vd.CommandGraph()
-        vd.fft.convolve2DR(buf, kb, kernel_inner_only=True, graph=cmd)
-        torch.cuda.synchronize()
-        g = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(g):
-            with vd.cuda_graph_capture(torch.cuda.current_stream()):
-                cmd.submit()
+    # warm-up outside the capture: the plan allocates its scratch on first use
+    x_t[..., :n] = torch.from_numpy(xs[0]).to("cuda")
+    torch.cuda.synchronize()
+    vd.fft.convolve2DR(buf, kb, kernel_inner_only=True)
+    vd.synchronize()
+    assert_close(x_t[..., :n].cpu().numpy(), O.convolve2DR(xs[0], K, kernel_inner_only=True), rel=1e-5)
+    cmd = vd.CommandGraph()
+    vd.fft.convolve2DR(buf, kb, kernel_inner_only=True, graph=cmd)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        with vd.cuda_graph_capture(torch.cuda.current_stream()):
+            cmd.submit()
     side = vd.asbuffer((rng.standard_normal((4, 256)) + 0j).astype(np.complex64))
     for rep in range(3):
         for x in xs:

@@ -365,6 +365,13 @@ int vkx_compile(const char* source, const char* name, const char* const* options
   return 0;
 }
 
+int vkx_compiler_version(int* version) {
+  // hiprtc reports only its API level (the same across ROCm releases); the runtime it
+  // ships with names the release, and both come from the same installation
+  VKX_HIP(hipRuntimeGetVersion(version));
+  return 0;
+}
+
 void vkx_free_host(void* p) { std::free(p); }
 
 int vkx_module_load(uint64_t* module, const void* code, size_t code_size) {

@@ -151,22 +151,13 @@ class Conv2DPhases:
             dispatch(k, vals, graph)
 
 
-def _mode(batch: int, W: int) -> str:
-    """VKX_CONV2D_MODE ("phases", the only pair-packed form).  VKX_FUSED_CONV=0 selects
-    the reference's three-dispatch composition instead (see fused_eligible)."""
-    m = os.environ.get("VKX_CONV2D_MODE", "phases")
-    if m != "phases":
-        raise ValueError("VKX_CONV2D_MODE is 'phases' (the single-launch form was measured slower and removed)")
-    return m
-
-
 # phase-kernel tilings: for 512 the row kernels run one row per wave (16 waves, 64 VGPRs,
 # no spills), so two workgroups per CU hold 8 waves per SIMD while they read HBM
 _PHASE_TILING = {512: dict(waves=16, irow_waves=8)}
 
 
 @lru_cache(maxsize=32)
-def fused_plan(batch: int, H: int, W: int, normalize: bool = True, mode: str = "phases"):
+def fused_plan(batch: int, H: int, W: int, normalize: bool = True):
     return Conv2DPhases(batch, H, W, normalize, **_PHASE_TILING.get(W, {}))
 
 
@@ -191,7 +182,7 @@ def plan_for_shape(real_shape, kernel_map=None, transposed_kernel=False, kernel_
         return None
     if batch > 1 and not kernel_inner_only:
         return None  # the reference then reads one kernel spectrum per image
-    return fused_plan(batch, H, W, bool(normalize), _mode(batch, W))
+    return fused_plan(batch, H, W, bool(normalize))
 
 
 def fused_eligible(buffer, kernel, kernel_map=None, buffer_shape=None, transposed_kernel=False,

@@ -42,6 +42,18 @@ def cache_dir() -> str:
 
 
 _header_digest = None
+_compiler_tag = None
+
+
+def compiler_tag() -> str:
+    """Release of the HIP installation whose hiprtc compiles (torch's bundled copy when
+    torch is installed, else the system ROCm)."""
+    global _compiler_tag
+    if _compiler_tag is None:
+        v = ctypes.c_int()
+        rt.check(rt.lib().vkx_compiler_version(ctypes.byref(v)), "vkx_compiler_version")
+        _compiler_tag = f"hip-{v.value}"
+    return _compiler_tag
 
 
 def _header_hash() -> str:
@@ -63,6 +75,7 @@ def source_key(source: str) -> str:
     h.update(_header_hash().encode())
     h.update(arch().encode())
     h.update(" ".join(JIT_FLAGS).encode())
+    h.update(compiler_tag().encode())
     return h.hexdigest()[:40]
 
 

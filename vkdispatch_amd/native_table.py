@@ -49,6 +49,13 @@ def _L():
             if _lib is None:
                 path = os.environ.get("VKX_LIB") or os.path.join(
                     os.path.dirname(os.path.abspath(__file__)), "_lib", "libvkdfft.so")
+                # torch's wheel carries its own HIP runtime under the same SONAMEs: loaded
+                # first, it is the one libvkdfft.so binds to (one runtime per process; a
+                # second copy loaded later finds no GPU)
+                try:
+                    import torch  # noqa: F401
+                except ImportError:
+                    pass
                 L = ctypes.CDLL(path)
                 L.vkx_last_error.restype = ctypes.c_char_p
                 L.vkx_compile.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_char_p),

@@ -82,6 +82,7 @@ _SIGS = {
     "vkx_compile": ([ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_char_p), ctypes.c_int,
                      ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_size_t),
                      ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
+    "vkx_compiler_version": ([ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "vkx_free_host": ([ctypes.c_void_p], None),
     "vkx_module_load": ([_PU64, ctypes.c_void_p, ctypes.c_size_t], ctypes.c_int),
     "vkx_module_unload": ([_U64], ctypes.c_int),
@@ -113,6 +114,21 @@ def lib_path() -> str:
     return os.environ.get("VKX_LIB", _build.LIB_PATH)
 
 
+def share_torch_hip_runtime() -> None:
+    """Import torch, when it is installed, before libvkdfft.so is loaded.
+
+    torch's ROCm wheel carries its own libamdhip64 / libhsa-runtime64 / libhiprtc under
+    the SONAMEs libvkdfft.so links against.  Already loaded, they are the copies
+    libvkdfft.so binds to, so the process has one HIP runtime and torch tensors, streams
+    and graph captures are valid handles here.  Loaded after libvkdfft.so (which then
+    brings the system ROCm copies), they form a second runtime in the same process, and
+    that one finds no GPU: torch raises "No HIP GPUs are available"."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def lib():
     """Load libvkdfft.so (once).  Raises RuntimeError if it is not built."""
     global _lib
@@ -125,6 +141,7 @@ def lib():
                 raise RuntimeError(
                     f"vkdispatch_amd native runtime not built ({path} missing); "
                     "run `python -m vkdispatch_amd._build` or __graft_entry__.build()")
+            share_torch_hip_runtime()
             L = ctypes.CDLL(path)
             for name, (argtypes, restype) in _SIGS.items():
                 fn = getattr(L, name)
