@@ -11,11 +11,14 @@ not feed results back into inputs.
 
 Output: one JSON line with the metric (GB/s of algorithmic bytes: 64 MiB in,
 64 MiB out, 1 MiB kernel spectrum per step), the roofline of the dominant
-kernel (HIP events on the launch stream), the FP64-oracle parity of the
-result, and the CPU baseline (the FP64 oracle, single core, on a bounded
-sample; plus multithreaded pocketfft via scipy).
+kernel (HIP events attached to its dispatches on the launch stream), the
+FP64-oracle parity of all 64 images, and the CPU baseline (multithreaded
+pocketfft via scipy on the whole workload; single-threaded numpy beside it).
+Without --only the line also carries the BF16-MFMA 256^3 ratio and the C2C,
+differentiable-convolution and ragged-batch workloads, each with its parity and
+CPU baseline.
 
-Other workloads (``--workload c2c4096``) print their own line for reference.
+Other workloads (``--workload c2c4096`` ...) print their own line.
 """
 from __future__ import annotations
 
@@ -186,28 +189,23 @@ def bench_conv(steps, warmup, args):
     ref = O.convolve2DR(x, K, kernel_inner_only=True)
     per_img = [O.rel_l2(res[b], ref[b]) for b in range(batch)]
 
-    # 2. per-kernel times in the same fresh-input loop: events between the launches
+    # 2. per-kernel times in the same fresh-input loop: each launch carries its own
+    # start / stop timestamps (hipExtModuleLaunchKernel events), so the launches stay
+    # back to back with no marker between them, as in the timed loop
     restore()
-    vals = plan.values(bufs[0], kbuf) if plan is not None else None
-    evs = [[vd.Event() for _ in range(len(kernels) + 1)] for _ in range(steps)]
-    for i in range(warmup):
-        run_direct(i)
-    for j, i in enumerate(range(warmup, warmup + steps)):
-        evs[j][0].record()
-        if plan is not None:
-            vals = plan.values(bufs[i], kbuf)
-            for kk, ev in zip(kernels, evs[j][1:]):
-                kk.launch(vals)
-                ev.record()
-        else:
-            run_direct(i)
-            evs[j][-1].record()
-    vd.synchronize()
     per_kernel = []
     if plan is not None:
+        evs = [[(vd.Event(), vd.Event()) for _ in kernels] for _ in range(steps)]
+        for i in range(warmup):
+            run_direct(i)
+        for j, i in enumerate(range(warmup, warmup + steps)):
+            vals = plan.values(bufs[i], kbuf)
+            for kk, (ea, eb) in zip(kernels, evs[j]):
+                kk.launch_timed(vals, ea, eb)
+        vd.synchronize()
         bytes_of = _conv_kernel_bytes(plan, batch, n)
         for kidx, name in enumerate(names):
-            d = [evs[j][kidx].elapsed_ms(evs[j][kidx + 1]) for j in range(steps)]
+            d = [evs[j][kidx][0].elapsed_ms(evs[j][kidx][1]) for j in range(steps)]
             per_kernel.append({"name": name, "ms_mean": float(np.mean(d)), "ms_median": float(np.median(d)),
                                "algorithmic_bytes": bytes_of[kidx]})
     dom = max(per_kernel, key=lambda p: p["ms_mean"]) if per_kernel else None
@@ -236,7 +234,9 @@ def bench_conv(steps, warmup, args):
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": committed_traffic(dom["name"]),
                     "kernel": dom["name"], "kernel_ms": round(dom["ms_mean"], 5),
                     "kernel_algorithmic_bytes": dom["algorithmic_bytes"],
-                    "timing": "HIP events on the launch stream around each launch of the timed fresh-input loop"}
+                    "timing": "mean over the steps of a fresh-input loop like the timed one; each launch "
+                              "records its own start / stop in HIP events on the launch stream "
+                              "(hipExtModuleLaunchKernel), no marker between launches"}
     out = {
         "metric": "FFT-conv GB/s & %HBM roofline; BF16-MFMA vs FP32 speedup at rel-L2<=1e-3",
         "value": round(value, 2), "unit": "GB/s", "n_gpus": 1, "steps": steps, "warmup": warmup,

@@ -140,6 +140,12 @@ int vkx_plan_destroy(uint64_t plan);
  * (vkdispatch/fft/io_manager.py:86-102); consts: packed Var/Const values. */
 int vkx_plan_exec(uint64_t plan, const uint64_t* bufs, int num_bufs, const void* consts, size_t consts_size,
                   uint64_t stream);
+/* vkx_plan_exec with the dispatch's own start / stop timestamps in two events
+ * (hipExtModuleLaunchKernel): per-kernel times inside a stream of back-to-back
+ * launches, without a marker between the launches.  No counterpart in the
+ * reference. */
+int vkx_plan_exec_timed(uint64_t plan, const uint64_t* bufs, int num_bufs, const void* consts, size_t consts_size,
+                        uint64_t stream, uint64_t start_event, uint64_t stop_event);
 
 /* ---- twiddles and chirps (generated in FP64 on the GPU) ----------------------- */
 /* Stockham twiddle table for a pass list: entry (NS_p - 1) + (r-1)*NS_p + k holds

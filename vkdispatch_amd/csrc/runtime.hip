@@ -6,6 +6,7 @@
 // are generated here in FP64 (sincospi in double, rounded once to the storage
 // precision).
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <hip/hiprtc.h>
 
 #include <cstdint>
@@ -513,12 +514,17 @@ int vkx_plan_destroy(uint64_t plan) {
   return 0;
 }
 
-int vkx_plan_exec(uint64_t plan, const uint64_t* bufs, int num_bufs, const void* consts, size_t consts_size,
-                  uint64_t stream) {
+}  // extern "C"
+
+namespace {
+// One launch of a plan; with start / stop events the dispatch itself carries the
+// timestamps (hipExtModuleLaunchKernel), so no marker sits between launches.
+int plan_launch(const char* where, uint64_t plan, const uint64_t* bufs, int num_bufs, const void* consts,
+                size_t consts_size, uint64_t stream, hipEvent_t start, hipEvent_t stop) {
   const Plan* p = reinterpret_cast<const Plan*>(plan);
-  if (!p) return fail("vkx_plan_exec", "null plan");
-  if (num_bufs < 0 || num_bufs > kMaxUserBufs) return fail("vkx_plan_exec", "too many buffers (max 12)");
-  if (consts_size > (size_t)kParamBytes) return fail("vkx_plan_exec", "constants exceed 256 bytes");
+  if (!p) return fail(where, "null plan");
+  if (num_bufs < 0 || num_bufs > kMaxUserBufs) return fail(where, "too many buffers (max 12)");
+  if (consts_size > (size_t)kParamBytes) return fail(where, "constants exceed 256 bytes");
   KParams prm;
   std::memset(&prm, 0, sizeof(prm));
   for (int i = 0; i < num_bufs; ++i) prm.p[i] = bufs[i];
@@ -528,9 +534,31 @@ int vkx_plan_exec(uint64_t plan, const uint64_t* bufs, int num_bufs, const void*
   if (consts_size) std::memcpy(prm.c, consts, consts_size);
   size_t sz = sizeof(prm);
   void* cfg[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, &prm, HIP_LAUNCH_PARAM_BUFFER_SIZE, &sz, HIP_LAUNCH_PARAM_END};
-  VKX_HIP(hipModuleLaunchKernel(p->fn, p->grid[0], p->grid[1], p->grid[2], p->block[0], p->block[1], p->block[2],
-                                p->shared, S(stream), nullptr, cfg));
-  return 0;
+  hipError_t e;
+  if (start || stop) {
+    e = hipExtModuleLaunchKernel(p->fn, p->grid[0] * p->block[0], p->grid[1] * p->block[1], p->grid[2] * p->block[2],
+                                 p->block[0], p->block[1], p->block[2], p->shared, S(stream), nullptr, cfg, start,
+                                 stop, 0);
+  } else {
+    e = hipModuleLaunchKernel(p->fn, p->grid[0], p->grid[1], p->grid[2], p->block[0], p->block[1], p->block[2],
+                              p->shared, S(stream), nullptr, cfg);
+  }
+  return e == hipSuccess ? 0 : hip_fail(where, e);
+}
+}  // namespace
+
+extern "C" {
+
+int vkx_plan_exec(uint64_t plan, const uint64_t* bufs, int num_bufs, const void* consts, size_t consts_size,
+                  uint64_t stream) {
+  return plan_launch("vkx_plan_exec", plan, bufs, num_bufs, consts, consts_size, stream, nullptr, nullptr);
+}
+
+int vkx_plan_exec_timed(uint64_t plan, const uint64_t* bufs, int num_bufs, const void* consts, size_t consts_size,
+                        uint64_t stream, uint64_t start_event, uint64_t stop_event) {
+  if (!start_event || !stop_event) return fail("vkx_plan_exec_timed", "null event");
+  return plan_launch("vkx_plan_exec_timed", plan, bufs, num_bufs, consts, consts_size, stream,
+                     reinterpret_cast<hipEvent_t>(start_event), reinterpret_cast<hipEvent_t>(stop_event));
 }
 
 int vkx_stream_is_capturing(uint64_t stream, int* capturing) {

@@ -330,6 +330,17 @@ class CompiledKernel:
         s = rt.current_stream() if stream is None else int(stream)
         rt.check(rt.lib().vkx_plan_exec(h, arr, len(ptrs), cbuf, len(consts), s), f"{self.name} launch")
 
+    def launch_timed(self, values: Sequence, start, stop, stream: Optional[int] = None) -> None:
+        """One launch whose dispatch records its own start / stop times in two
+        runtime Events (no marker between this launch and its neighbours)."""
+        ptrs, consts = self.bind(values)
+        h = self.handle()
+        arr = (ctypes.c_uint64 * max(1, len(ptrs)))(*ptrs)
+        cbuf = ctypes.create_string_buffer(consts, max(1, len(consts)))
+        s = rt.current_stream() if stream is None else int(stream)
+        rt.check(rt.lib().vkx_plan_exec_timed(h, arr, len(ptrs), cbuf, len(consts), s, start.handle, stop.handle),
+                 f"{self.name} timed launch")
+
 
 class SpillError(RuntimeError):
     """A hiprtc kernel needs private (scratch) memory for register spills."""

@@ -93,6 +93,8 @@ _SIGS = {
     "vkx_plan_create": ([_PU64, ctypes.POINTER(PlanDesc)], ctypes.c_int),
     "vkx_plan_destroy": ([_U64], ctypes.c_int),
     "vkx_plan_exec": ([_U64, _PU64, ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t, _U64], ctypes.c_int),
+    "vkx_plan_exec_timed": ([_U64, _PU64, ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t, _U64, _U64, _U64],
+                            ctypes.c_int),
     "vkx_twiddles": ([_U64, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int, _U64],
                      ctypes.c_int),
     "vkx_roots": ([_U64, ctypes.c_int, ctypes.c_int, _U64], ctypes.c_int),
