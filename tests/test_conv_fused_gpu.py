@@ -151,3 +151,28 @@ def test_headline_workload_every_image():
     worst = max(O.rel_l2(got[b], ref[b]) for b in range(64))
     print(f"  worst per-image rel-L2 {worst:.3e}, whole batch {O.rel_l2(got, ref):.3e}", flush=True)
     assert worst <= 1e-5
+
+
+def test_timed_launches_match_plain_launches():
+    """vkx_plan_exec_timed (each dispatch records its own start / stop events, as
+    bench.py times the kernels) computes exactly what the plain launches do and
+    reports a positive duration for every kernel."""
+    print("pair-packed convolve2DR through timed launches", flush=True)
+    rng = np.random.default_rng(21)
+    n = 512
+    x = rng.uniform(-1, 1, (4, n, n)).astype(np.float32)
+    K = np.fft.rfft2(_gauss_kernel(rng, n).astype(np.float64)).astype(np.complex64)
+    plain = _run(x, K)
+    buf = vd.asrfftbuffer(x)
+    kb = vd.RFFTBuffer((n, n))
+    kb.write_fourier(K)
+    plan = fused.fused_eligible(buf, kb, kernel_inner_only=True)
+    vals = plan.values(buf, kb)
+    evs = [(vd.Event(), vd.Event()) for _ in plan.kernels()]
+    for kk, (a, b) in zip(plan.kernels(), evs):
+        kk.launch_timed(vals, a, b)
+    vd.synchronize()
+    assert np.array_equal(buf.read_real(0), plain)
+    ms = [a.elapsed_ms(b) for a, b in evs]
+    print(f"  kernel ms {ms}", flush=True)
+    assert all(0.0 < t < 100.0 for t in ms)

@@ -56,6 +56,10 @@ def test_plan_create_and_exec_reject_null_handles():
     assert "null function" in _err()
     assert rt.lib().vkx_plan_exec(0, None, 0, None, 0, 0) == 1
     assert "null plan" in _err()
+    assert rt.lib().vkx_plan_exec_timed(0, None, 0, None, 0, 0, 0, 0) == 1
+    assert "vkx_plan_exec_timed: null event" in _err()
+    assert rt.lib().vkx_plan_exec_timed(0, None, 0, None, 0, 0, 1, 2) == 1
+    assert "vkx_plan_exec_timed: null plan" in _err()
 
 
 def _compile(source: str, opts=("--offload-arch=gfx950",)):
