@@ -336,7 +336,7 @@ def bench_ragged(steps, warmup, args):
     buf = vd.Buffer((packed.size,), vd.complex64)
     tuned = R.tune(sizes, buf, reps=3, cache=bool(getattr(args, "cached_tune", False)))
     best = tuned["best"]
-    plan = R.RaggedFFT(sizes, **{k: best[k] for k in R.TUNE_KEYS})
+    plan = R.RaggedFFT(sizes, **R.tuned_args(best))
     g = vd.CommandGraph()
     g.record_callable(plan.bind(buf, src))
     g.capture()

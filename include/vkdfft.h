@@ -81,6 +81,10 @@ int vkx_device_synchronize(void);
 int vkx_event_create(uint64_t* event);
 int vkx_event_destroy(uint64_t event);
 int vkx_event_record(uint64_t event, uint64_t stream);
+/* work queued on `stream` after this call waits for `event` (replaces a queue waiting
+ * on another queue's signal, native.signal_wait, vkdispatch/backends/opencl_backend.py:1372);
+ * also joins a second stream into a graph capture of the stream that recorded `event` */
+int vkx_stream_wait_event(uint64_t stream, uint64_t event);
 int vkx_event_synchronize(uint64_t event);
 int vkx_event_elapsed_ms(float* ms, uint64_t start, uint64_t stop);
 

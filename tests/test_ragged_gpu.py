@@ -113,6 +113,50 @@ def test_ragged_long_and_bluestein_both_paths(four_step, sync):
     assert O.rel_l2(dst.read(0), packed) < 1e-5
 
 
+def test_ragged_length_classes_forward_inverse_and_graph():
+    """Power-of-two lengths with at least CLASS_MIN_COUNT transforms run as length
+    classes (vd.fft kernels with gather / scatter hooks; two launches for 32768 and
+    65536), interleaved in the packed buffer with items of the persistent kernel:
+    forward out of place, graph replays, inverse in place, against FP64."""
+    rng = np.random.default_rng(17)
+    sizes = [8192] * 16 + [16384] * 16 + [32768] * 16 + [65536] * 16 + [100, 4099, 60060, 8192, 12288] * 4
+    sizes = [int(n) for n in rng.permutation(sizes)]
+    parts, packed = _packed(sizes, 17)
+    src = vd.asbuffer(packed)
+    dst = vd.Buffer((packed.size,), vd.complex64)
+    plan = R.RaggedFFT(sizes)
+    print(f"length classes {[c.L for c in plan._classes]}, {plan.n_items} persistent-kernel items", flush=True)
+    assert sorted(c.L for c in plan._classes) == [8192, 16384, 32768, 65536]
+    assert {int(n) for n in plan._items["n"]} == {100, 4099, 60060, 12288}  # every 8192 is in its class
+    plan(dst, src)
+    assert not plan.aborted()
+    _check(sizes, parts, dst.read(0))
+    assert np.array_equal(src.read(0), packed)
+    g = vd.CommandGraph()
+    g.record_callable(plan.bind(dst, src))
+    for _ in range(3):
+        dst.write(np.zeros_like(packed))
+        g.submit()
+        vd.synchronize()
+        _check(sizes, parts, dst.read(0))
+    inv = R.RaggedFFT(sizes, inverse=True)
+    inv(dst)
+    assert not inv.aborted()
+    assert O.rel_l2(dst.read(0), packed) < 1e-5
+
+
+def test_uniform_long_batch_runs_as_a_length_class():
+    """vd.fft.fft on (32, 65536): the length is beyond the single-workgroup kernels, the
+    batch is one length class."""
+    rng = np.random.default_rng(23)
+    x = (rng.standard_normal((32, 65536)) + 1j * rng.standard_normal((32, 65536))).astype(np.complex64)
+    buf = vd.asbuffer(x)
+    vd.fft.fft(buf)
+    assert O.rel_l2(buf.read(0), np.fft.fft(x.astype(np.complex128), axis=1)) < 1e-5
+    vd.fft.ifft(buf)
+    assert O.rel_l2(buf.read(0), x) < 1e-5
+
+
 def test_ragged_wait_gives_up_and_the_launch_ends():
     """An item whose wait target is never reached (set from the host) polls its
     counter spin_limit times, sets the abort flag and the launch still ends."""
@@ -139,7 +183,7 @@ def test_tune_is_cached_per_batch(tmp_path):
     buf = vd.asbuffer(packed)
     path = str(tmp_path / "tuned.json")
     first = R.tune(sizes, buf, reps=1, cache=True, save=True, path=path)
-    assert not first["cached"] and len(first["results"]) == 6 * len(R.TUNE_TILE_MIN)
+    assert not first["cached"] and len(first["results"]) == 2 * 6 * len(R.TUNE_TILE_MIN)  # classes on / off
     again = R.tune(sizes, buf, reps=1, cache=True, path=path)
     assert again["cached"] and {k: again["best"][k] for k in R.TUNE_KEYS} == {k: first["best"][k] for k in R.TUNE_KEYS}
 

@@ -215,3 +215,53 @@ def test_auto_tile_min():
     assert R.auto_tile_min([8192] * 32, 512) > 8192  # short lengths stay whole
     assert R.auto_tile_min(R.mixed_sizes(4096, seed=0), 512) == R.TILE_MIN_DIRECT
     assert R.auto_tile_min([65536] * 600, 512) == R.TILE_MIN_DIRECT  # enough long transforms to fill the GPU
+
+
+def test_class_groups_pick_frequent_power_of_two_lengths():
+    sizes = [8192] * 16 + [16384] * 15 + [65536] * 20 + [4096] * 40 + [12288] * 30 + [32768] * 16
+    g = R.class_groups(sizes, 16)
+    assert sorted(g) == [8192, 32768, 65536]
+    assert list(g[8192]) == list(range(16)) and len(g[65536]) == 20
+    assert R.class_groups(sizes, 0) == {}
+
+
+@pytest.mark.parametrize("L", [32768, 65536])
+@pytest.mark.parametrize("inverse", [False, True])
+def test_length_class_split_computes_the_dft(L, inverse):
+    """The two-launch length class as its hooks index it (csrc-independent restatement
+    of _class_hooks): transform b is gathered from offs[b], column transforms of length
+    n1 over j1 (stride n2), times tw[k1 j2] (exp(-+2 pi i m / L)), row transforms of
+    length n2, scattered to offs[b] + k1 + n1 k2; each stage normalised by its own
+    length on the inverse.  Checked against numpy.fft on a packed batch with gaps."""
+    n1, n2 = R.CLASS_SPLIT[L]
+    rng = np.random.default_rng(L % 97 + inverse)
+    count = 3
+    offs = np.array([5, 5 + L + 7, 5 + 2 * L + 20])
+    total = offs[-1] + L + 3
+    buf = rng.standard_normal(total) + 1j * rng.standard_normal(total)
+    m = np.arange(L)
+    tw = np.exp((2j if inverse else -2j) * np.pi * m / L)
+    io = np.arange(count * L)
+    b, r = io // L, io % L
+    x = buf[offs[b] + r].reshape(count, n1, n2)                      # gather
+    y = (np.fft.ifft(x, axis=1) if inverse else np.fft.fft(x, axis=1)).reshape(-1)
+    y = y * tw[(r // n2) * (r % n2)]                                   # twiddle store
+    z = (np.fft.ifft(y.reshape(count, n1, n2), axis=2) if inverse
+         else np.fft.fft(y.reshape(count, n1, n2), axis=2)).reshape(-1)
+    out = np.zeros(total, complex)
+    out[offs[b] + r // n2 + (r % n2) * n1] = z                          # transposed scatter
+    for k in range(count):
+        seg = buf[offs[k]:offs[k] + L]
+        ref = np.fft.ifft(seg) if inverse else np.fft.fft(seg)
+        got = out[offs[k]:offs[k] + L]
+        assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-12
+
+
+@pytest.mark.parametrize("L", [8192, 16384, 32768, 65536])
+def test_length_class_kernels_compile(L):
+    """Each class's kernels (gather / scatter hooks, twiddle store) build for gfx950
+    without spills."""
+    from vkdispatch_amd.fft import kernels as K
+    for k in R.class_kernels(L, 17) + R.class_kernels(L, 17, inverse=True):
+        assert K.compile_source(k.source)[:4] == b"\x7fELF"
+        assert k.code_object_info()["private_bytes"] == 0, k.name

@@ -326,6 +326,11 @@ int vkx_event_record(uint64_t event, uint64_t stream) {
   return 0;
 }
 
+int vkx_stream_wait_event(uint64_t stream, uint64_t event) {
+  VKX_HIP(hipStreamWaitEvent(S(stream), reinterpret_cast<hipEvent_t>(event), 0));
+  return 0;
+}
+
 int vkx_event_synchronize(uint64_t event) {
   VKX_HIP(hipEventSynchronize(reinterpret_cast<hipEvent_t>(event)));
   return 0;

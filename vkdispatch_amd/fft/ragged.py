@@ -41,6 +41,7 @@ from typing import Dict, List, NamedTuple, Optional, Sequence, Tuple
 
 import numpy as np
 
+from .. import codegen as vc
 from .. import runtime as rt
 from .prime_utils import prime_factors
 
@@ -62,6 +63,11 @@ LDS_TILE = 4096 + 128  # complex64 elements of one LDS buffer (a tile at pitch n
 FOUR_STEP_MAX = 512  # longest four-step factor: tiles of >= 8 padded columns fit one LDS buffer
 TILE_MIN_DIRECT = 1 << 17  # direct lengths from here on are tiled (shorter ones run whole on one workgroup)
 DIRECT_PRIMES = (2, 3, 5, 7, 11, 13)
+# power-of-two length classes (LengthClass): the lengths, the two-launch splits, and how
+# many transforms of a length make a class
+CLASS_LENGTHS = (8192, 16384, 32768, 65536)
+CLASS_SPLIT = {32768: (128, 256), 65536: (256, 256)}  # (n1, n2): column length, row length
+CLASS_MIN_COUNT = 16  # fewer transforms of a length stay items of the persistent kernel
 
 
 def runtime_radices(n: int, max_radix: int = 16) -> List[int]:
@@ -300,7 +306,8 @@ class ItemQueue(NamedTuple):
 
 
 def plan_items(sizes: Sequence[int], inverse: bool, normalize: bool, mr: int, four_step: bool, T,
-               sync: str = "launch", tile_min: int = TILE_MIN_DIRECT, fill: int = 0) -> ItemQueue:
+               sync: str = "launch", tile_min: int = TILE_MIN_DIRECT, fill: int = 0,
+               offsets: Optional[Sequence[int]] = None) -> ItemQueue:
     """Work-item queue of a ragged batch.  ``T`` provides the device tables
     (twiddles, twiddles_split, chirp_table, chirp_spectrum) as addresses.
     sync="launch" (default): one launch per tile level (the kernel boundary orders the
@@ -308,11 +315,15 @@ def plan_items(sizes: Sequence[int], inverse: bool, normalize: bool, mr: int, fo
     sync="counter": one launch, later tile levels wait on done counters (each item's
     agent-scope release / acquire writes back and invalidates L2, which costs more
     than the launches it saves on this 8-XCD device).  Direct lengths >= tile_min and
-    Bluestein lengths are tiled when four_step is set and the length splits."""
+    Bluestein lengths are tiled when four_step is set and the length splits.
+    offsets: element offset of each transform in the packed buffers (default: packed
+    back to back in the order given)."""
     if sync not in ("counter", "launch"):
         raise ValueError("sync is 'counter' or 'launch'")
     sizes = np.asarray(sizes, dtype=np.int64)
-    offsets = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    if offsets is None:
+        offsets = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    offsets = np.asarray(offsets, dtype=np.int64)
 
     def counter() -> int:
         nctr[0] += 1
@@ -425,12 +436,14 @@ class RaggedFFT:
     <= 512 run as column-tile and row-tile items spread over all workgroups; without
     it they run whole on one workgroup through a scratch slice.  ``fill``: whole
     transforms of at most this length fill the tails of the later tile launches
-    (4096-transform mix: 1191 -> 1168 us)."""
+    (4096-transform mix: 1191 -> 1168 us).  ``classes``: a power-of-two length of
+    CLASS_LENGTHS that occurs at least this many times runs as one length class
+    (LengthClass) instead of as items of the persistent kernel; 0 turns classes off."""
 
     def __init__(self, sizes: Sequence[int], inverse: bool = False, normalize: bool = True, block: int = 256,
                  max_radix: int = 16, grid: Optional[int] = None, four_step: bool = True, spin_limit: int = 0,
                  sync: str = "launch", tile_min: Optional[int] = None, block_first: Optional[int] = None,
-                 fill: int = LDS_PINGPONG):
+                 fill: int = LDS_PINGPONG, classes: int = CLASS_MIN_COUNT):
         self.sizes = np.asarray(sizes, dtype=np.int64)
         if np.any(self.sizes < 1):
             raise ValueError("transform lengths must be positive")
@@ -447,8 +460,15 @@ class RaggedFFT:
             tile_min = auto_tile_min(self.sizes, 2 * int(dev.compute_units))
         self.tile_min = int(tile_min)
         self.sync = sync
-        q = plan_items(self.sizes, self.inverse, self.normalize, self.max_radix, four_step, tables(), sync,
-                       self.tile_min, int(fill))
+        self.classes = int(classes)
+        groups = class_groups(self.sizes, self.classes)
+        rest = np.ones(len(self.sizes), bool)
+        for idx in groups.values():
+            rest[idx] = False
+        self._classes = [LengthClass(L, self.offsets[idx], self.inverse, self.normalize)
+                         for L, idx in sorted(groups.items(), reverse=True)]
+        q = plan_items(self.sizes[rest], self.inverse, self.normalize, self.max_radix, four_step, tables(), sync,
+                       self.tile_min, int(fill), offsets=self.offsets[rest])
         self._items, self._src, self._dst = q.items, q.src, q.dst
         self._segments = q.segments
         self._nctr = q.ncounters
@@ -489,21 +509,40 @@ class RaggedFFT:
                 raise ValueError(f"buffer of {getattr(b, 'size', 0)} elements, ragged batch needs {self.total}")
         if any(int(n) == 1 for n in self.sizes) and in_buf is not out_buf:
             raise ValueError("out-of-place ragged batches may not contain length-1 transforms")
-        tp = self._tasks_for(int(in_buf.ptr), int(out_buf.ptr))
+        tp = self._tasks_for(int(in_buf.ptr), int(out_buf.ptr)) if len(self._items) else 0
         if not self._wg_scr:  # per-workgroup scratch of whole transforms, owned by the plan
             self._wg_scr = rt.malloc(max(1, self.scratch_per_wg) * 8 * self.grid)
         wg_scr = self._wg_scr
-        keep = (out_buf, in_buf)
+        class_runs = [c.bind(out_buf, in_buf) for c in self._classes]
+        # the length classes and the persistent kernel's items touch disjoint transforms:
+        # the classes run on the side stream, forked from and joined back into the
+        # caller's stream, so they fill the CUs the persistent launches leave idle
+        fork = join = side = None
+        if class_runs and self._segments:
+            fork, join, side = rt.Event(), rt.Event(), rt.side_stream()
+        keep = (out_buf, in_buf, fork, join)
 
         def launch(stream=None):
             _ = keep
+            s = rt.current_stream() if stream is None else int(stream)
+            if side is not None:
+                fork.record(s)
+                rt.stream_wait(side, fork)
+                for run in class_runs:
+                    run(side)
+                join.record(side)
+            else:
+                for run in class_runs:
+                    run(s)
             if self.sync == "counter":
-                rt.note_async_check(self, rt.current_stream() if stream is None else stream)
+                rt.note_async_check(self, s)
             for i, (a, b) in enumerate(self._segments):
                 _launch_tasks(self._items[a:b], self.scratch_per_wg, self.grid, self.block_first if i == 0 else self.block,
-                              stream,
+                              s,
                               task_buf=tp + a * RTASK_DTYPE.itemsize, counters=(self._ctr, self._nctr),
                               spin_limit=self.spin_limit, scratch=wg_scr)
+            if side is not None:
+                rt.stream_wait(s, join)
         return launch
 
     def __call__(self, out_buf, in_buf=None, stream=None) -> None:
@@ -596,6 +635,141 @@ def mixed_sizes(count: int, seed: int = 0, lo: int = 64, hi: int = 65536) -> Lis
     return out
 
 
+# ---- power-of-two length classes -------------------------------------------------
+# A batch with many transforms of one power-of-two length runs them through the
+# specialised kernels of vd.fft instead of through the persistent kernel: all of them
+# as one virtual (count, L) batch whose read and write hooks map transform b to its
+# offset in the packed buffers.  Lengths up to 16384 take one single-workgroup kernel
+# (the whole transform in LDS, one pass over memory); 32768 and 65536 take two: column
+# transforms of length n1 (strided by n2) with the twiddles W_L^(k1 j2) applied in the
+# write hook, into a plan-owned scratch batch, then row transforms of length n2 written
+# to X[k1 + n1 k2].  The persistent kernel's whole-transform path crosses memory once
+# per radix pass (four times for these lengths); measured on uniform batches of the
+# 4096-transform mix, the class kernels take 15.5 / 21.1 / 42.4 / 81.8 us where the
+# persistent kernel takes 47.7 / 72.6 / 127.7 / 347.8 us (8192 x 235, 16384 x 210,
+# 32768 x 195, 65536 x 236).
+
+
+def class_groups(sizes: Sequence[int], min_count: int) -> Dict[int, np.ndarray]:
+    """Indices of the transforms of each length in CLASS_LENGTHS that occurs at least
+    ``min_count`` times (no classes when min_count <= 0)."""
+    if min_count <= 0:
+        return {}
+    sizes = np.asarray(sizes)
+    groups = {}
+    for L in CLASS_LENGTHS:
+        idx = np.nonzero(sizes == L)[0]
+        if len(idx) >= min_count:
+            groups[L] = idx
+    return groups
+
+
+def _class_hooks(L: int, n1: int, n2: int):
+    """Read / write hooks of a length class: gather (transform b's element r at
+    offs[b] + r), twiddle store (y[b][k1][j2] = v W^(k1 j2) from a length-L table) and
+    transposed scatter (X[offs[b] + k1 + n1 k2])."""
+    from ..shader_map import map as vd_map
+    from .hooks import read_op, write_op
+
+    @vd_map
+    def gather(x: vc.Buffer[vc.c64], offs: vc.Buffer[vc.i32]):
+        op = read_op()
+        io = op.io_index
+        op.read_from_buffer(x, io_index=offs[io // L] + io % L)
+
+    @vd_map
+    def scatter(out: vc.Buffer[vc.c64], offs: vc.Buffer[vc.i32]):
+        op = write_op()
+        io = op.io_index
+        op.write_to_buffer(out, io_index=offs[io // L] + io % L)
+
+    @vd_map
+    def twiddle_store(y: vc.Buffer[vc.c64], tw: vc.Buffer[vc.c64]):
+        op = write_op()
+        r = op.io_index % L
+        w = vc.new_register(vc.c64)
+        w[:] = tw[(r // n2) * (r % n2)]
+        op.register[:] = vc.mult_complex(op.register, w)
+        op.write_to_buffer(y)
+
+    @vd_map
+    def scatter_t(out: vc.Buffer[vc.c64], offs: vc.Buffer[vc.i32]):
+        op = write_op()
+        io = op.io_index
+        r = io % L
+        op.write_to_buffer(out, io_index=offs[io // L] + r // n2 + (r % n2) * n1)
+
+    return gather, scatter, twiddle_store, scatter_t
+
+
+def class_kernels(L: int, count: int, inverse: bool = False, normalize: bool = True) -> list:
+    """The vd.fft kernels of a length class of ``count`` transforms of length L (no
+    device needed: the code objects can be built ahead of time)."""
+    from .. import dtypes as dt
+    from .shader_factories import make_fft_shader
+    c64 = dt.complex64
+    common = dict(inverse=bool(inverse), normalize_inverse=bool(normalize), input_type=c64, output_type=c64,
+                  compute_type=c64)
+    if L not in CLASS_SPLIT:
+        gather, scatter, _, _ = _class_hooks(L, L, 1)
+        return [make_fft_shader((count, L), 1, input_map=gather, output_map=scatter, **common)]
+    n1, n2 = CLASS_SPLIT[L]
+    gather, _, twiddle_store, scatter_t = _class_hooks(L, n1, n2)
+    shape = (count, n1, n2)
+    return [make_fft_shader(shape, 1, input_map=gather, output_map=twiddle_store, **common),
+            make_fft_shader(shape, 2, output_map=scatter_t, **common)]
+
+
+class LengthClass:
+    """All transforms of one power-of-two length L of a ragged batch (their element
+    offsets ``offsets`` in the packed buffers), run through vd.fft kernels; see
+    CLASS_LENGTHS.  bind() returns a launch(stream) that allocates nothing."""
+
+    def __init__(self, L: int, offsets: np.ndarray, inverse: bool, normalize: bool):
+        from .. import dtypes as dt
+        from ..buffer import Buffer
+        self.L, self.count = int(L), len(offsets)
+        if self.L not in CLASS_LENGTHS:
+            raise ValueError(f"no length class for {L}")
+        if int(np.max(offsets)) + self.L > 2 ** 31 - 1:
+            raise ValueError("length-class offsets must fit int32")
+        self.offs = Buffer((self.count,), dt.int32)
+        self.offs.write(np.asarray(offsets, np.int32))
+        self.kernels = class_kernels(self.L, self.count, inverse, normalize)
+        c64 = dt.complex64
+        if self.L not in CLASS_SPLIT:
+            self.scratch = self.tw = None
+        else:
+            self.scratch = Buffer((self.count * self.L,), c64)
+            # W_L^(+-m) for m < L, generated in FP64 on the GPU (the inverse uses the conjugate)
+            self.tw = Buffer((self.L,), c64)
+            rt.check(rt.lib().vkx_roots(self.tw.ptr, self.L, 0, rt.current_stream()), "vkx_roots")
+            if inverse:
+                t = self.tw.read(0)
+                self.tw.write(np.conj(t).astype(np.complex64))
+
+    def values(self, out_buf, in_buf) -> List[list]:
+        if self.scratch is None:
+            return [[out_buf, self.offs, in_buf, self.offs]]
+        return [[self.scratch, self.tw, in_buf, self.offs], [out_buf, self.offs, self.scratch]]
+
+    def bind(self, out_buf, in_buf):
+        """Pointers and constants of every launch for this (output, input) pair, resolved
+        now (code objects loaded, spill check done), so the returned launch(stream)
+        makes only launch calls (safe under graph capture)."""
+        packed = []
+        for k, vals in zip(self.kernels, self.values(out_buf, in_buf)):
+            k.handle()
+            packed.append((k,) + tuple(k.bind(vals)))
+        keep = (out_buf, in_buf, self)
+
+        def launch(stream=None):
+            _ = keep
+            for k, ptrs, consts in packed:
+                k.launch_raw(ptrs, consts, stream)
+        return launch
+
+
 def packed_normal(sizes: Sequence[int], seed: int = 0):
     """Per-transform N(0,1) real then imaginary parts, in size order, packed complex64."""
     rng = np.random.default_rng(seed)
@@ -604,7 +778,7 @@ def packed_normal(sizes: Sequence[int], seed: int = 0):
 
 
 TUNE_TILE_MIN = (4097, 16385, 32769, 65536, TILE_MIN_DIRECT)  # direct lengths tiled from ... / never
-TUNE_KEYS = ("block", "block_first", "max_radix", "tile_min")  # RaggedFFT arguments chosen by tune()
+TUNE_KEYS = ("block", "block_first", "max_radix", "tile_min", "classes")  # RaggedFFT arguments chosen by tune()
 TUNE_TABLE_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_ragged.json")
 
 
@@ -629,11 +803,12 @@ def save_tune_table(table: Dict, path: str = TUNE_TABLE_PATH) -> None:
 def tune(sizes: Sequence[int], buf, reps: int = 5, cache: bool = False, save: bool = False,
          path: str = TUNE_TABLE_PATH) -> Dict:
     """Pick the workgroup sizes (first launch, which holds the whole transforms, and
-    the tile launches), the largest radix and the direct length from which
-    transforms are tiled for this batch by timing each candidate plan (deterministic
-    candidate order; ties keep the earlier candidate).  The best entry's keys are
-    RaggedFFT arguments.  cache: return the entry stored for (device arch, batch) in
-    the table at ``path`` without timing, when there is one; save: store the result."""
+    the tile launches), the largest radix, the direct length from which transforms
+    are tiled, and whether power-of-two length classes are used, for this batch by
+    timing each candidate plan (deterministic candidate order; ties keep the earlier
+    candidate).  The best entry's keys are RaggedFFT arguments.  cache: return the
+    entry stored for (device arch, batch) in the table at ``path`` without timing,
+    when there is one; save: store the result."""
     key = f"{rt.device_arch()}|{batch_key(sizes)}"
     if cache:
         hit = load_tune_table(path).get(key)
@@ -641,24 +816,26 @@ def tune(sizes: Sequence[int], buf, reps: int = 5, cache: bool = False, save: bo
             return {"best": dict(hit), "results": [], "cached": True}
     best = None
     results = []
-    for block, block_first in ((256, 256), (256, 512), (512, 512)):
-        for max_radix in (8, 16):
-            for tile_min in TUNE_TILE_MIN:
-                plan = RaggedFFT(sizes, block=block, max_radix=max_radix, tile_min=tile_min, block_first=block_first)
-                run = plan.bind(buf)
-                run()
-                e0, e1 = rt.Event(), rt.Event()
-                e0.record()
-                for _ in range(reps):
+    for classes in (CLASS_MIN_COUNT, 0):
+        for block, block_first in ((256, 256), (256, 512), (512, 512)):
+            for max_radix in (8, 16):
+                for tile_min in TUNE_TILE_MIN:
+                    plan = RaggedFFT(sizes, block=block, max_radix=max_radix, tile_min=tile_min,
+                                     block_first=block_first, classes=classes)
+                    run = plan.bind(buf)
                     run()
-                e1.record()
-                e1.synchronize()
-                ms = e0.elapsed_ms(e1) / reps
-                results.append({"block": block, "block_first": block_first, "max_radix": max_radix,
-                                "tile_min": tile_min, "ms": ms})
-                if best is None or ms < best["ms"]:
-                    best = results[-1]
-                del run, plan
+                    e0, e1 = rt.Event(), rt.Event()
+                    e0.record()
+                    for _ in range(reps):
+                        run()
+                    e1.record()
+                    e1.synchronize()
+                    ms = e0.elapsed_ms(e1) / reps
+                    results.append({"block": block, "block_first": block_first, "max_radix": max_radix,
+                                    "tile_min": tile_min, "classes": classes, "ms": ms})
+                    if best is None or ms < best["ms"]:
+                        best = results[-1]
+                    del run, plan
     if save:
         table = load_tune_table(path)
         table[key] = {k: best[k] for k in TUNE_KEYS}
@@ -669,4 +846,10 @@ def tune(sizes: Sequence[int], buf, reps: int = 5, cache: bool = False, save: bo
 def tuned_plan(sizes: Sequence[int], buf, inverse: bool = False, normalize: bool = True) -> "RaggedFFT":
     """RaggedFFT with the cached (or freshly measured and stored) tuning for this batch."""
     best = tune(sizes, buf, cache=True, save=True)["best"]
-    return RaggedFFT(sizes, inverse=inverse, normalize=normalize, **{k: best[k] for k in TUNE_KEYS})
+    return RaggedFFT(sizes, inverse=inverse, normalize=normalize, **tuned_args(best))
+
+
+def tuned_args(best: Dict) -> Dict:
+    """RaggedFFT keyword arguments of a tuning entry (keys an older entry lacks keep
+    their defaults)."""
+    return {k: best[k] for k in TUNE_KEYS if k in best}

@@ -77,6 +77,7 @@ _SIGS = {
     "vkx_event_create": ([_PU64], ctypes.c_int),
     "vkx_event_destroy": ([_U64], ctypes.c_int),
     "vkx_event_record": ([_U64, _U64], ctypes.c_int),
+    "vkx_stream_wait_event": ([_U64, _U64], ctypes.c_int),
     "vkx_event_synchronize": ([_U64], ctypes.c_int),
     "vkx_event_elapsed_ms": ([ctypes.POINTER(ctypes.c_float), _U64, _U64], ctypes.c_int),
     "vkx_compile": ([ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_char_p), ctypes.c_int,
@@ -217,6 +218,29 @@ def current_stream() -> int:
         return _state.stream_override
     ensure_device()
     return _default_stream
+
+
+_side_stream = None
+
+
+def side_stream() -> int:
+    """A second library stream (default priority), created once per process.  A plan
+    whose work splits into independent parts forks one part onto it and joins back with
+    events (stream_wait), inside a graph capture too."""
+    global _side_stream
+    if _side_stream is None:
+        ensure_device()
+        with _device_lock:
+            if _side_stream is None:
+                s = ctypes.c_uint64(0)
+                check(lib().vkx_stream_create(ctypes.byref(s)), "vkx_stream_create")
+                _side_stream = s.value
+    return _side_stream
+
+
+def stream_wait(stream: int, event: "Event") -> None:
+    """Work queued on ``stream`` from now on waits for ``event``."""
+    check(lib().vkx_stream_wait_event(int(stream), event.handle), "vkx_stream_wait_event")
 
 
 def set_stream(stream) -> None:
