@@ -145,6 +145,49 @@ def test_ragged_length_classes_forward_inverse_and_graph():
     assert O.rel_l2(dst.read(0), packed) < 1e-5
 
 
+def _primes_between(lo, hi, count, rng):
+    sieve = np.ones(hi + 1, bool)
+    sieve[:2] = False
+    for i in range(2, int(hi ** 0.5) + 1):
+        if sieve[i]:
+            sieve[i * i::i] = False
+    ps = np.nonzero(sieve)[0]
+    ps = ps[ps > lo]
+    return [int(p) for p in rng.choice(ps, count, replace=False)]
+
+
+def test_ragged_bluestein_classes_forward_inverse_and_graph():
+    """Bluestein transforms sharing a size m (16 each for m = 8192, 65536 and 131072)
+    run as Bluestein classes (chirped gather, spectrum multiply, chirped output around
+    two FFT_m of the vd.fft kernels), next to items of the persistent kernel: forward
+    out of place, graph replays, inverse in place, against FP64."""
+    rng = np.random.default_rng(29)
+    blue = _primes_between(2048, 4096, 16, rng) + _primes_between(16384, 32768, 16, rng) + \
+        _primes_between(32768, 65536, 16, rng)
+    sizes = [int(n) for n in rng.permutation(blue + [100, 4099, 60060, 12288, 97] * 3)]
+    parts, packed = _packed(sizes, 29)
+    src = vd.asbuffer(packed)
+    dst = vd.Buffer((packed.size,), vd.complex64)
+    plan = R.RaggedFFT(sizes)
+    ms = sorted(c.m for c in plan._classes if isinstance(c, R.BlueClass))
+    print(f"Bluestein classes {ms}, {plan.n_items} persistent-kernel items", flush=True)
+    assert ms == [8192, 65536, 131072]
+    plan(dst, src)
+    assert not plan.aborted()
+    _check(sizes, parts, dst.read(0))
+    g = vd.CommandGraph()
+    g.record_callable(plan.bind(dst, src))
+    for _ in range(2):
+        dst.write(np.zeros_like(packed))
+        g.submit()
+        vd.synchronize()
+        _check(sizes, parts, dst.read(0))
+    inv = R.RaggedFFT(sizes, inverse=True)
+    inv(dst)
+    assert not inv.aborted()
+    assert O.rel_l2(dst.read(0), packed) < 1e-5
+
+
 def test_uniform_long_batch_runs_as_a_length_class():
     """vd.fft.fft on (32, 65536): the length is beyond the single-workgroup kernels, the
     batch is one length class."""

@@ -239,13 +239,12 @@ def test_length_class_split_computes_the_dft(L, inverse):
     offs = np.array([5, 5 + L + 7, 5 + 2 * L + 20])
     total = offs[-1] + L + 3
     buf = rng.standard_normal(total) + 1j * rng.standard_normal(total)
-    m = np.arange(L)
-    tw = np.exp((2j if inverse else -2j) * np.pi * m / L)
+    tw = R.split_twiddles(L, inverse).astype(complex)
     io = np.arange(count * L)
     b, r = io // L, io % L
     x = buf[offs[b] + r].reshape(count, n1, n2)                      # gather
     y = (np.fft.ifft(x, axis=1) if inverse else np.fft.fft(x, axis=1)).reshape(-1)
-    y = y * tw[(r // n2) * (r % n2)]                                   # twiddle store
+    y = y * tw[r]                                                      # twiddle store at its own position
     z = (np.fft.ifft(y.reshape(count, n1, n2), axis=2) if inverse
          else np.fft.fft(y.reshape(count, n1, n2), axis=2)).reshape(-1)
     out = np.zeros(total, complex)
@@ -254,7 +253,7 @@ def test_length_class_split_computes_the_dft(L, inverse):
         seg = buf[offs[k]:offs[k] + L]
         ref = np.fft.ifft(seg) if inverse else np.fft.fft(seg)
         got = out[offs[k]:offs[k] + L]
-        assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-12
+        assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 2e-7  # complex64 twiddle table
 
 
 @pytest.mark.parametrize("L", [8192, 16384, 32768, 65536])
@@ -263,5 +262,70 @@ def test_length_class_kernels_compile(L):
     without spills."""
     from vkdispatch_amd.fft import kernels as K
     for k in R.class_kernels(L, 17) + R.class_kernels(L, 17, inverse=True):
+        assert K.compile_source(k.source)[:4] == b"\x7fELF"
+        assert k.code_object_info()["private_bytes"] == 0, k.name
+
+
+def test_blue_class_groups_pick_frequent_bluestein_sizes():
+    sizes = [4093] * 16 + [4099] * 9 + [8191] * 11 + [65521] * 3 + [30011] * 16 + [4096] * 50
+    g = R.blue_class_groups(sizes, 16)
+    assert sorted(g) == [8192, 16384, 65536]  # m >= 2n - 1: 4093 -> 8192; 4099, 8191 -> 16384; 30011 -> 65536
+    assert len(g[16384]) == 20 and R.blue_class_groups(sizes, 0) == {}
+
+
+@pytest.mark.parametrize("m,ns", [(8192, (4093, 4001, 2053)), (131072, (65521, 40009, 32771))])
+@pytest.mark.parametrize("inverse", [False, True])
+def test_bluestein_class_pipeline_computes_the_dft(m, ns, inverse):
+    """A Bluestein class launch by launch as its hooks compute it (restated with numpy):
+    chirped, clamped and masked gather; forward FFT_m (for a split size: columns over
+    j1, twiddles read at their own position, rows over j2, output left in the rows
+    order r = k1 n2 + k2); times the spectrum packed in that order (conjugated and
+    reflected on the inverse); inverse FFT_m from that order (rows, conjugate twiddles,
+    columns), normalised by 1/m, which lands in natural order; the first n outputs
+    times the (conjugate) chirp and the transform's scale.  Against numpy.fft for
+    primes with gaps between the transforms in the packed buffer."""
+    rng = np.random.default_rng(m % 89 + inverse)
+    sizes = np.array(ns)
+    assert all(R.bluestein_length(int(n)) == m for n in sizes)
+    offs = np.concatenate([[3], 3 + np.cumsum(sizes + 5)[:-1]])
+    total = int(offs[-1] + sizes[-1] + 2)
+    x = rng.standard_normal(total) + 1j * rng.standard_normal(total)
+    count = len(sizes)
+    io = np.arange(count * m)
+    b, j = io // m, io % m
+    jj = np.minimum(j, sizes[b] - 1)
+    w = [_chirp(int(n), inverse) for n in sizes]
+    a = np.array([x[offs[bb] + q] * w[bb][q] for bb, q in zip(b, jj)])
+    a[j >= sizes[b]] = 0
+    order = R.blue_spectrum_order(m)
+    if inverse:
+        order = (m - order) % m
+    B = np.stack([_kernel_spectrum(int(n), m) for n in sizes])
+    bs = np.conj(B[:, order]) if inverse else B[:, order]
+    if m not in R.CLASS_SPLIT:
+        C = np.fft.fft(a.reshape(count, m), axis=1) * bs
+        c = np.fft.ifft(C, axis=1)
+    else:
+        n1, n2 = R.CLASS_SPLIT[m]
+        tw, twi = R.split_twiddles(m).astype(complex), R.split_twiddles(m, inverse=True).astype(complex)
+        y = np.fft.fft(a.reshape(count, n1, n2), axis=1).reshape(count, m) * tw
+        C = np.fft.fft(y.reshape(count, n1, n2), axis=2).reshape(count, m) * bs
+        y = np.fft.ifft(C.reshape(count, n1, n2), axis=2).reshape(count, m) * twi
+        c = np.fft.ifft(y.reshape(count, n1, n2), axis=1).reshape(count, m)
+    out = np.zeros(total, complex)
+    for bb, n in enumerate(sizes):
+        scale = 1.0 / n if inverse else 1.0
+        out[offs[bb]:offs[bb] + n] = c[bb, :n] * w[bb][:n] * scale
+    for bb, n in enumerate(sizes):
+        seg = x[offs[bb]:offs[bb] + n]
+        ref = np.fft.ifft(seg) if inverse else np.fft.fft(seg)
+        got = out[offs[bb]:offs[bb] + n]
+        assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 2e-7  # complex64 tables
+
+
+@pytest.mark.parametrize("m", [8192, 32768, 131072])
+def test_bluestein_class_kernels_compile(m):
+    from vkdispatch_amd.fft import kernels as K
+    for k in R.blue_class_kernels(m, 17) + R.blue_class_kernels(m, 17, inverse=True):
         assert K.compile_source(k.source)[:4] == b"\x7fELF"
         assert k.code_object_info()["private_bytes"] == 0, k.name

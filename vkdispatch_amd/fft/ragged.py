@@ -66,8 +66,9 @@ DIRECT_PRIMES = (2, 3, 5, 7, 11, 13)
 # power-of-two length classes (LengthClass): the lengths, the two-launch splits, and how
 # many transforms of a length make a class
 CLASS_LENGTHS = (8192, 16384, 32768, 65536)
-CLASS_SPLIT = {32768: (128, 256), 65536: (256, 256)}  # (n1, n2): column length, row length
+CLASS_SPLIT = {32768: (128, 256), 65536: (256, 256), 131072: (256, 512)}  # (n1, n2): column, row length
 CLASS_MIN_COUNT = 16  # fewer transforms of a length stay items of the persistent kernel
+BLUE_CLASS_LENGTHS = (8192, 16384, 32768, 65536, 131072)  # Bluestein sizes m of Bluestein classes
 
 
 def runtime_radices(n: int, max_radix: int = 16) -> List[int]:
@@ -462,11 +463,14 @@ class RaggedFFT:
         self.sync = sync
         self.classes = int(classes)
         groups = class_groups(self.sizes, self.classes)
+        bgroups = blue_class_groups(self.sizes, self.classes)
         rest = np.ones(len(self.sizes), bool)
-        for idx in groups.values():
+        for idx in list(groups.values()) + list(bgroups.values()):
             rest[idx] = False
-        self._classes = [LengthClass(L, self.offsets[idx], self.inverse, self.normalize)
-                         for L, idx in sorted(groups.items(), reverse=True)]
+        self._classes = [BlueClass(m, self.sizes[idx], self.offsets[idx], self.inverse, self.normalize, self.max_radix)
+                         for m, idx in sorted(bgroups.items(), reverse=True)]
+        self._classes += [LengthClass(L, self.offsets[idx], self.inverse, self.normalize)
+                          for L, idx in sorted(groups.items(), reverse=True)]
         q = plan_items(self.sizes[rest], self.inverse, self.normalize, self.max_radix, four_step, tables(), sync,
                        self.tile_min, int(fill), offsets=self.offsets[rest])
         self._items, self._src, self._dst = q.items, q.src, q.dst
@@ -686,9 +690,8 @@ def _class_hooks(L: int, n1: int, n2: int):
     @vd_map
     def twiddle_store(y: vc.Buffer[vc.c64], tw: vc.Buffer[vc.c64]):
         op = write_op()
-        r = op.io_index % L
         w = vc.new_register(vc.c64)
-        w[:] = tw[(r // n2) * (r % n2)]
+        w[:] = tw[op.io_index % L]  # split_twiddles: W^((r / n2) (r % n2)) at r, read in order
         op.register[:] = vc.mult_complex(op.register, w)
         op.write_to_buffer(y)
 
@@ -700,6 +703,16 @@ def _class_hooks(L: int, n1: int, n2: int):
         op.write_to_buffer(out, io_index=offs[io // L] + r // n2 + (r % n2) * n1)
 
     return gather, scatter, twiddle_store, scatter_t
+
+
+def split_twiddles(L: int, inverse: bool = False) -> np.ndarray:
+    """Twiddles of the split (n1, n2) = CLASS_SPLIT[L] in the order a launch over the
+    (n1, n2) view meets them: entry r = a n2 + c holds exp(-+2 pi i a c / L) (FP64,
+    rounded once), so the write hook reads the table at its own position, in order."""
+    n1, n2 = CLASS_SPLIT[L]
+    r = np.arange(L, dtype=np.int64)
+    e = ((r // n2) * (r % n2)) % L
+    return np.exp((2j if inverse else -2j) * np.pi * e / L).astype(np.complex64)
 
 
 def class_kernels(L: int, count: int, inverse: bool = False, normalize: bool = True) -> list:
@@ -743,10 +756,7 @@ class LengthClass:
             self.scratch = Buffer((self.count * self.L,), c64)
             # W_L^(+-m) for m < L, generated in FP64 on the GPU (the inverse uses the conjugate)
             self.tw = Buffer((self.L,), c64)
-            rt.check(rt.lib().vkx_roots(self.tw.ptr, self.L, 0, rt.current_stream()), "vkx_roots")
-            if inverse:
-                t = self.tw.read(0)
-                self.tw.write(np.conj(t).astype(np.complex64))
+            self.tw.write(split_twiddles(self.L, inverse))
 
     def values(self, out_buf, in_buf) -> List[list]:
         if self.scratch is None:
@@ -757,6 +767,195 @@ class LengthClass:
         """Pointers and constants of every launch for this (output, input) pair, resolved
         now (code objects loaded, spill check done), so the returned launch(stream)
         makes only launch calls (safe under graph capture)."""
+        packed = []
+        for k, vals in zip(self.kernels, self.values(out_buf, in_buf)):
+            k.handle()
+            packed.append((k,) + tuple(k.bind(vals)))
+        keep = (out_buf, in_buf, self)
+
+        def launch(stream=None):
+            _ = keep
+            for k, ptrs, consts in packed:
+                k.launch_raw(ptrs, consts, stream)
+        return launch
+
+
+# ---- Bluestein length classes -------------------------------------------------------
+# Lengths with a prime factor above 13 (X_k = w_k sum_j x_j w_j conj(w_(k-j)),
+# w_j = exp(-i pi j^2 / n)) of one Bluestein size m (the power of two >= 2n - 1) run as
+# one class: a forward FFT_m of the chirped, zero-padded inputs times the chirp
+# spectrum B (per transform, packed [count][m]), then an inverse FFT_m whose write
+# hook keeps the first n outputs times the chirp.  The same semantics as the
+# persistent kernel's Bluestein items (csrc/ragged.hip run_task): FP64 chirp tables,
+# an inverse transform conjugates the chirp and uses conj(B[(m - k) mod m]).  m up to
+# 16384 takes one launch per FFT_m, larger m two (the length-class split), so a class
+# is two or four launches.
+
+
+def blue_class_groups(sizes: Sequence[int], min_count: int) -> Dict[int, np.ndarray]:
+    """Indices of the Bluestein transforms of each size m in BLUE_CLASS_LENGTHS shared by
+    at least ``min_count`` transforms (no classes when min_count <= 0)."""
+    if min_count <= 0:
+        return {}
+    sizes = np.asarray(sizes)
+    ms = np.array([0 if is_direct(int(n)) else bluestein_length(int(n)) for n in sizes])
+    groups = {}
+    for m in BLUE_CLASS_LENGTHS:
+        idx = np.nonzero(ms == m)[0]
+        if len(idx) >= min_count:
+            groups[m] = idx
+    return groups
+
+
+def _blue_hooks(m: int, inverse: bool):
+    """Hooks of a Bluestein class of size m: chirped gather (j < n: x[offs + j] w_j, else
+    0; indices clamped in range), spectrum multiply (bs[b m + r], the spectrum already in
+    the order the forward rows launch produces it, conjugated and reflected for an
+    inverse transform) and the chirped output (k < n: out[offs + k] = c_k w_k scale).
+    Every hook reads and writes at its own position r = io mod m: the forward FFT_m
+    leaves its output in the rows launch's order, the inverse FFT_m starts with rows on
+    that order and ends with columns, whose positions are the natural time order."""
+    from ..shader_map import map as vd_map
+    from .hooks import read_op, write_op
+
+    def chirp(ch, coff, b, j):
+        w = vc.new_register(vc.c64)
+        w[:] = ch[coff[b] + j]
+        return w.conjugate() if inverse else w
+
+    @vd_map
+    def gather(x: vc.Buffer[vc.c64], offs: vc.Buffer[vc.i32], nn: vc.Buffer[vc.i32], ch: vc.Buffer[vc.c64],
+               coff: vc.Buffer[vc.i32]):
+        op = read_op()
+        io = op.io_index
+        b = (io // m).to_register()
+        j = (io % m).to_register()
+        n = nn[b].to_register()
+        jj = vc.min(j, n - 1).to_register()
+        op.read_from_buffer(x, io_index=offs[b] + jj)
+        op.register[:] = vc.mult_complex(op.register, chirp(ch, coff, b, jj))
+        with vc.if_block(j >= n):
+            op.register[:] = vc.to_complex(0.0, 0.0)
+
+    @vd_map
+    def spec_store(y: vc.Buffer[vc.c64], bs: vc.Buffer[vc.c64]):
+        op = write_op()
+        v = vc.new_register(vc.c64)
+        v[:] = bs[op.io_index]
+        op.register[:] = vc.mult_complex(op.register, v)
+        op.write_to_buffer(y)
+
+    @vd_map
+    def chirp_out(out: vc.Buffer[vc.c64], offs: vc.Buffer[vc.i32], nn: vc.Buffer[vc.i32], ch: vc.Buffer[vc.c64],
+                  coff: vc.Buffer[vc.i32], scl: vc.Buffer[vc.f32]):
+        op = write_op()
+        io = op.io_index
+        b = (io // m).to_register()
+        k = (io % m).to_register()
+        with vc.if_block(k < nn[b]):
+            op.register[:] = vc.mult_complex(op.register, chirp(ch, coff, b, k)) * scl[b]
+            op.write_to_buffer(out, io_index=offs[b] + k)
+
+    return gather, spec_store, chirp_out
+
+
+def blue_spectrum_order(m: int) -> np.ndarray:
+    """Frequency index held at position r of a class's forward FFT_m output: r itself
+    for one launch; k1 + n1 k2 at r = k1 n2 + k2 after the columns and rows launches."""
+    r = np.arange(m, dtype=np.int64)
+    if m not in CLASS_SPLIT:
+        return r
+    n1, n2 = CLASS_SPLIT[m]
+    return r // n2 + (r % n2) * n1
+
+
+def blue_class_kernels(m: int, count: int, inverse: bool = False) -> list:
+    """The vd.fft kernels of a Bluestein class (no device needed)."""
+    from .. import dtypes as dt
+    from .shader_factories import make_fft_shader
+    c64 = dt.complex64
+    fwd = dict(inverse=False, input_type=c64, output_type=c64, compute_type=c64)
+    bwd = dict(inverse=True, normalize_inverse=True, input_type=c64, output_type=c64, compute_type=c64)
+    gather, spec_store, chirp_out = _blue_hooks(m, inverse)
+    if m not in CLASS_SPLIT:
+        return [make_fft_shader((count, m), 1, input_map=gather, output_map=spec_store, **fwd),
+                make_fft_shader((count, m), 1, output_map=chirp_out, **bwd)]
+    n1, n2 = CLASS_SPLIT[m]
+    _, _, tw_store, _ = _class_hooks(m, n1, n2)
+    shape = (count, n1, n2)
+    # forward: columns over j1 (j = j1 n2 + j2), twiddles, rows over j2 -> C at
+    # r = k1 n2 + k2 (k = k1 + n1 k2); inverse from that order: rows over k2, conjugate
+    # twiddles, columns over k1 -> c at r = j1' n2 + j2' = its natural index
+    return [make_fft_shader(shape, 1, input_map=gather, output_map=tw_store, **fwd),
+            make_fft_shader(shape, 2, output_map=spec_store, **fwd),
+            make_fft_shader(shape, 2, output_map=tw_store, **bwd),
+            make_fft_shader(shape, 1, output_map=chirp_out, **bwd)]
+
+
+class BlueClass:
+    """The Bluestein transforms (lengths ``sizes``, element offsets ``offsets``) of one
+    Bluestein size m of a ragged batch, run through vd.fft kernels; see
+    BLUE_CLASS_LENGTHS.  bind() returns a launch(stream) that allocates nothing."""
+
+    def __init__(self, m: int, sizes: np.ndarray, offsets: np.ndarray, inverse: bool, normalize: bool,
+                 max_radix: int = 16):
+        from .. import dtypes as dt
+        from ..buffer import Buffer
+        self.m, self.count = int(m), len(sizes)
+        sizes = np.asarray(sizes, np.int64)
+        if self.m not in BLUE_CLASS_LENGTHS or any(bluestein_length(int(n)) != self.m for n in sizes):
+            raise ValueError(f"not a Bluestein class of size {m}")
+        if int(np.max(offsets + sizes)) > 2 ** 31 - 1 or int(sizes.sum()) > 2 ** 31 - 1:
+            raise ValueError("Bluestein-class offsets must fit int32")
+        T = tables()
+        c64 = dt.complex64
+        self.offs = Buffer((self.count,), dt.int32)
+        self.offs.write(np.asarray(offsets, np.int32))
+        self.nn = Buffer((self.count,), dt.int32)
+        self.nn.write(sizes.astype(np.int32))
+        coff = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int32)
+        self.coff = Buffer((self.count,), dt.int32)
+        self.coff.write(coff)
+        self.scl = Buffer((self.count,), dt.float32)
+        self.scl.write(((1.0 / sizes) if (inverse and normalize) else np.ones(self.count)).astype(np.float32))
+        # packed FP64-generated chirps (sum of n); chirp spectra ([count][m]) in the order
+        # the forward FFT_m leaves its output, conjugated and reflected for an inverse
+        self.ch = Buffer((int(sizes.sum()),), c64)
+        rads = runtime_radices(self.m, max_radix)
+        s = rt.current_stream()
+        order = blue_spectrum_order(self.m)
+        if inverse:
+            order = (self.m - order) % self.m
+        spec = np.empty((self.count, self.m), np.complex64)
+        for i, n in enumerate(sizes.tolist()):
+            rt.dtod(self.ch.ptr + int(coff[i]) * 8, T.chirp_table(n), n * 8, s)
+            B = np.empty(self.m, np.complex64)
+            rt.dtoh(B.ctypes.data, T.chirp_spectrum(n, self.m, rads), self.m * 8, s)
+            rt.synchronize(s)
+            spec[i] = np.conj(B[order]) if inverse else B[order]
+        self.bs = Buffer((self.count * self.m,), c64)
+        self.bs.write(spec.reshape(-1))
+        self.kernels = blue_class_kernels(self.m, self.count, inverse)
+        self.y = Buffer((self.count * self.m,), c64)
+        self.z = self.tw = self.twi = None
+        if self.m in CLASS_SPLIT:
+            self.z = Buffer((self.count * self.m,), c64)
+            self.tw = Buffer((self.m,), c64)
+            self.tw.write(split_twiddles(self.m))
+            self.twi = Buffer((self.m,), c64)
+            self.twi.write(split_twiddles(self.m, inverse=True))
+        rt.synchronize()
+
+    def values(self, out_buf, in_buf) -> List[list]:
+        io = [self.offs, self.nn, self.ch, self.coff]
+        if self.m not in CLASS_SPLIT:
+            return [[self.y, self.bs, in_buf] + io, [out_buf] + io + [self.scl, self.y]]
+        return [[self.y, self.tw, in_buf] + io,          # chirped columns, twiddles
+                [self.z, self.bs, self.y],               # rows, times the spectrum (rows order)
+                [self.y, self.twi, self.z],              # inverse rows, conjugate twiddles
+                [out_buf] + io + [self.scl, self.y]]     # inverse columns, chirped first n outputs
+
+    def bind(self, out_buf, in_buf):
         packed = []
         for k, vals in zip(self.kernels, self.values(out_buf, in_buf)):
             k.handle()
