@@ -218,9 +218,9 @@ def test_auto_tile_min():
 
 
 def test_class_groups_pick_frequent_power_of_two_lengths():
-    sizes = [8192] * 16 + [16384] * 15 + [65536] * 20 + [4096] * 40 + [12288] * 30 + [32768] * 16
+    sizes = [8192] * 16 + [16384] * 15 + [65536] * 20 + [4096] * 40 + [12288] * 30 + [32768] * 16 + [32] * 20
     g = R.class_groups(sizes, 16)
-    assert sorted(g) == [8192, 32768, 65536]
+    assert sorted(g) == [4096, 8192, 32768, 65536]  # 16384: too few; 12288: not a power of two; 32: too short
     assert list(g[8192]) == list(range(16)) and len(g[65536]) == 20
     assert R.class_groups(sizes, 0) == {}
 
@@ -276,14 +276,17 @@ def test_blue_class_groups_pick_frequent_bluestein_sizes():
 @pytest.mark.parametrize("m,ns", [(8192, (4093, 4001, 2053)), (131072, (65521, 40009, 32771))])
 @pytest.mark.parametrize("inverse", [False, True])
 def test_bluestein_class_pipeline_computes_the_dft(m, ns, inverse):
-    """A Bluestein class launch by launch as its hooks compute it (restated with numpy):
-    chirped, clamped and masked gather; forward FFT_m (for a split size: columns over
-    j1, twiddles read at their own position, rows over j2, output left in the rows
-    order r = k1 n2 + k2); times the spectrum packed in that order (conjugated and
-    reflected on the inverse); inverse FFT_m from that order (rows, conjugate twiddles,
-    columns), normalised by 1/m, which lands in natural order; the first n outputs
-    times the (conjugate) chirp and the transform's scale.  Against numpy.fft for
-    primes with gaps between the transforms in the packed buffer."""
+    """A Bluestein class as its hooks compute it (restated with numpy): chirped, clamped
+    and masked gather; forward FFT_m (for a split size: columns over j1, twiddles read
+    at their own position, rows over j2, output left in the rows order r = k1 n2 + k2);
+    times the spectrum packed in that order (conjugated and reflected on the inverse;
+    the class stores its conjugate as the convolution kernel K of X . conj(K)); inverse
+    FFT_m from that order (rows, conjugate twiddles, columns), normalised by 1/m, which
+    lands in natural order; the first n outputs times the (conjugate) chirp and the
+    transform's scale.  One convolution launch for m <= 16384; for a split size the
+    rows FFT, the product and the inverse rows are one convolution launch between the
+    two column launches.  Against numpy.fft for primes with gaps between the
+    transforms in the packed buffer."""
     rng = np.random.default_rng(m % 89 + inverse)
     sizes = np.array(ns)
     assert all(R.bluestein_length(int(n)) == m for n in sizes)

@@ -22,8 +22,15 @@ into a queue of work items for one persistent launch of the ragged kernel
 With one launch the queue holds all first-level tiles, then each later level,
 then the whole transforms longest first; an item only waits for items queued
 before it.  With a launch per level the whole transforms head the first level's queue.
-``tune()`` picks the workgroup size, the largest radix and the direct length
-from which transforms are tiled by timing the actual batch; ``tuned_plan()``
+
+A length that many transforms share leaves the queue: power-of-two lengths of
+CLASS_LENGTHS (LengthClass) and Bluestein sizes m of BLUE_CLASS_LENGTHS
+(BlueClass) run as one virtual batch each through the vd.fft kernels, with
+gather / scatter hooks that map transform b to its offset in the packed
+buffers, on a side stream forked from and joined back into the caller's stream.
+``tune()`` picks the workgroup size, the largest radix, the direct length
+from which transforms are tiled and whether classes are used by timing the
+actual batch; ``tuned_plan()``
 keeps the pick in ``tuned_ragged.json`` (keyed by device arch and a hash of the
 length list) so a batch is timed once per device.
 
@@ -65,10 +72,11 @@ TILE_MIN_DIRECT = 1 << 17  # direct lengths from here on are tiled (shorter ones
 DIRECT_PRIMES = (2, 3, 5, 7, 11, 13)
 # power-of-two length classes (LengthClass): the lengths, the two-launch splits, and how
 # many transforms of a length make a class
-CLASS_LENGTHS = (8192, 16384, 32768, 65536)
+CLASS_LENGTHS = tuple(2 ** k for k in range(6, 17))  # 64 .. 65536
 CLASS_SPLIT = {32768: (128, 256), 65536: (256, 256), 131072: (256, 512)}  # (n1, n2): column, row length
 CLASS_MIN_COUNT = 16  # fewer transforms of a length stay items of the persistent kernel
-BLUE_CLASS_LENGTHS = (8192, 16384, 32768, 65536, 131072)  # Bluestein sizes m of Bluestein classes
+BLUE_CLASS_LENGTHS = tuple(2 ** k for k in range(7, 18))  # Bluestein sizes m of Bluestein classes: 128 .. 131072
+CLASS_STREAMS = 2  # side streams the classes are spread over
 
 
 def runtime_radices(n: int, max_radix: int = 16) -> List[int]:
@@ -439,12 +447,16 @@ class RaggedFFT:
     transforms of at most this length fill the tails of the later tile launches
     (4096-transform mix: 1191 -> 1168 us).  ``classes``: a power-of-two length of
     CLASS_LENGTHS that occurs at least this many times runs as one length class
-    (LengthClass) instead of as items of the persistent kernel; 0 turns classes off."""
+    (LengthClass), and a Bluestein size m of BLUE_CLASS_LENGTHS shared by that many as
+    one Bluestein class (BlueClass), instead of as items of the persistent kernel; 0
+    turns classes off.  ``class_streams``: side streams the classes are spread over;
+    ``persistent_first``: enqueue the persistent launches before the classes."""
 
     def __init__(self, sizes: Sequence[int], inverse: bool = False, normalize: bool = True, block: int = 256,
                  max_radix: int = 16, grid: Optional[int] = None, four_step: bool = True, spin_limit: int = 0,
                  sync: str = "launch", tile_min: Optional[int] = None, block_first: Optional[int] = None,
-                 fill: int = LDS_PINGPONG, classes: int = CLASS_MIN_COUNT):
+                 fill: int = LDS_PINGPONG, classes: int = CLASS_MIN_COUNT, class_streams: int = CLASS_STREAMS,
+                 persistent_first: bool = False):
         self.sizes = np.asarray(sizes, dtype=np.int64)
         if np.any(self.sizes < 1):
             raise ValueError("transform lengths must be positive")
@@ -462,6 +474,10 @@ class RaggedFFT:
         self.tile_min = int(tile_min)
         self.sync = sync
         self.classes = int(classes)
+        if int(class_streams) < 1:
+            raise ValueError("class_streams must be >= 1")
+        self.class_streams = int(class_streams)
+        self.persistent_first = bool(persistent_first)
         groups = class_groups(self.sizes, self.classes)
         bgroups = blue_class_groups(self.sizes, self.classes)
         rest = np.ones(len(self.sizes), bool)
@@ -517,35 +533,45 @@ class RaggedFFT:
         if not self._wg_scr:  # per-workgroup scratch of whole transforms, owned by the plan
             self._wg_scr = rt.malloc(max(1, self.scratch_per_wg) * 8 * self.grid)
         wg_scr = self._wg_scr
-        class_runs = [c.bind(out_buf, in_buf) for c in self._classes]
-        # the length classes and the persistent kernel's items touch disjoint transforms:
-        # the classes run on the side stream, forked from and joined back into the
-        # caller's stream, so they fill the CUs the persistent launches leave idle
-        fork = join = side = None
-        if class_runs and self._segments:
-            fork, join, side = rt.Event(), rt.Event(), rt.side_stream()
-        keep = (out_buf, in_buf, fork, join)
+        # the length classes and the persistent kernel's items touch disjoint transforms,
+        # and so do the classes among themselves: the classes run on side streams (lanes,
+        # balanced by the bytes they move), forked from and joined back into the caller's
+        # stream, so a class launch of few workgroups shares the GPU with the others and
+        # with the persistent launches
+        lanes = class_lanes(self._classes, self.class_streams)
+        runs = [[self._classes[i].bind(out_buf, in_buf) for i in lane] for lane in lanes]
+        side = rt.side_streams(len(runs)) if runs else []
+        fork = rt.Event() if runs else None
+        joins = [rt.Event() for _ in runs]
+        keep = (out_buf, in_buf, fork, joins)
 
         def launch(stream=None):
             _ = keep
             s = rt.current_stream() if stream is None else int(stream)
-            if side is not None:
+            if runs:
                 fork.record(s)
-                rt.stream_wait(side, fork)
-                for run in class_runs:
-                    run(side)
-                join.record(side)
-            else:
-                for run in class_runs:
-                    run(s)
-            if self.sync == "counter":
-                rt.note_async_check(self, s)
-            for i, (a, b) in enumerate(self._segments):
-                _launch_tasks(self._items[a:b], self.scratch_per_wg, self.grid, self.block_first if i == 0 else self.block,
-                              s,
-                              task_buf=tp + a * RTASK_DTYPE.itemsize, counters=(self._ctr, self._nctr),
-                              spin_limit=self.spin_limit, scratch=wg_scr)
-            if side is not None:
+
+            def persistent():
+                if self.sync == "counter":
+                    rt.note_async_check(self, s)
+                for i, (a, b) in enumerate(self._segments):
+                    _launch_tasks(self._items[a:b], self.scratch_per_wg, self.grid,
+                                  self.block_first if i == 0 else self.block, s,
+                                  task_buf=tp + a * RTASK_DTYPE.itemsize, counters=(self._ctr, self._nctr),
+                                  spin_limit=self.spin_limit, scratch=wg_scr)
+
+            # a replayed graph dispatches its nodes in the order they were recorded: the
+            # part enqueued second starts once the first part has been issued
+            if self.persistent_first:
+                persistent()
+            for lane_runs, ls, join in zip(runs, side, joins):
+                rt.stream_wait(ls, fork)
+                for run in lane_runs:
+                    run(ls)
+                join.record(ls)
+            if not self.persistent_first:
+                persistent()
+            for join in joins:
                 rt.stream_wait(s, join)
         return launch
 
@@ -654,6 +680,18 @@ def mixed_sizes(count: int, seed: int = 0, lo: int = 64, hi: int = 65536) -> Lis
 # 32768 x 195, 65536 x 236).
 
 
+def class_lanes(classes: Sequence, streams: int) -> List[List[int]]:
+    """Indices of ``classes`` per side stream: largest first (bytes_moved()), each to the
+    least-loaded of at most ``streams`` lanes; empty lanes are dropped."""
+    lanes: List[List[int]] = [[] for _ in range(max(1, int(streams)))]
+    load = [0] * len(lanes)
+    for i in sorted(range(len(classes)), key=lambda i: -classes[i].bytes_moved()):
+        j = min(range(len(lanes)), key=lambda j: (load[j], j))
+        lanes[j].append(i)
+        load[j] += classes[i].bytes_moved()
+    return [lane for lane in lanes if lane]
+
+
 def class_groups(sizes: Sequence[int], min_count: int) -> Dict[int, np.ndarray]:
     """Indices of the transforms of each length in CLASS_LENGTHS that occurs at least
     ``min_count`` times (no classes when min_count <= 0)."""
@@ -705,14 +743,34 @@ def _class_hooks(L: int, n1: int, n2: int):
     return gather, scatter, twiddle_store, scatter_t
 
 
+def split_exponents(L: int) -> np.ndarray:
+    """Exponent of W_L held at position r = a n2 + c of the split (n1, n2) =
+    CLASS_SPLIT[L] twiddle table: (a c) mod L."""
+    n1, n2 = CLASS_SPLIT[L]
+    r = np.arange(L, dtype=np.int64)
+    return ((r // n2) * (r % n2)) % L
+
+
 def split_twiddles(L: int, inverse: bool = False) -> np.ndarray:
     """Twiddles of the split (n1, n2) = CLASS_SPLIT[L] in the order a launch over the
     (n1, n2) view meets them: entry r = a n2 + c holds exp(-+2 pi i a c / L) (FP64,
-    rounded once), so the write hook reads the table at its own position, in order."""
-    n1, n2 = CLASS_SPLIT[L]
-    r = np.arange(L, dtype=np.int64)
-    e = ((r // n2) * (r % n2)) % L
-    return np.exp((2j if inverse else -2j) * np.pi * e / L).astype(np.complex64)
+    rounded once), so the write hook reads the table at its own position, in order.
+    Host statement of the table split_twiddle_buffer() builds on the device."""
+    return np.exp((2j if inverse else -2j) * np.pi * split_exponents(L) / L).astype(np.complex64)
+
+
+def split_twiddle_buffer(L: int, inverse: bool = False):
+    """Device table of split_twiddles(L, inverse): the roots W_L^m generated in FP64 on
+    the GPU (vkx_roots, rounded once to complex64), permuted to the split order (the
+    inverse takes their conjugates, which is exact)."""
+    from .. import dtypes as dt
+    from ..buffer import Buffer
+    roots = Buffer((L,), dt.complex64)
+    rt.check(rt.lib().vkx_roots(roots.ptr, L, 0, rt.current_stream()), "vkx_roots")
+    w = roots.read(0)[split_exponents(L)]
+    out = Buffer((L,), dt.complex64)
+    out.write(np.conj(w) if inverse else w)
+    return out
 
 
 def class_kernels(L: int, count: int, inverse: bool = False, normalize: bool = True) -> list:
@@ -755,8 +813,11 @@ class LengthClass:
         else:
             self.scratch = Buffer((self.count * self.L,), c64)
             # W_L^(+-m) for m < L, generated in FP64 on the GPU (the inverse uses the conjugate)
-            self.tw = Buffer((self.L,), c64)
-            self.tw.write(split_twiddles(self.L, inverse))
+            self.tw = split_twiddle_buffer(self.L, inverse)
+
+    def bytes_moved(self) -> int:
+        """Global-memory bytes of the class's launches (each reads and writes every point)."""
+        return 16 * self.count * self.L * (1 if self.scratch is None else 2)
 
     def values(self, out_buf, in_buf) -> List[list]:
         if self.scratch is None:
@@ -788,8 +849,11 @@ class LengthClass:
 # hook keeps the first n outputs times the chirp.  The same semantics as the
 # persistent kernel's Bluestein items (csrc/ragged.hip run_task): FP64 chirp tables,
 # an inverse transform conjugates the chirp and uses conj(B[(m - k) mod m]).  m up to
-# 16384 takes one launch per FFT_m, larger m two (the length-class split), so a class
-# is two or four launches.
+# 16384 is one convolution launch (vd.fft.convolve's kernel: chirped gather, FFT_m,
+# times the spectrum, inverse FFT_m, chirped output, all in LDS).  Larger m use the
+# length-class split and three launches: forward columns with the twiddles, then one
+# convolution along the rows (FFT over k2, times the spectrum, inverse FFT, conjugate
+# twiddles), then inverse columns with the chirped output.
 
 
 def blue_class_groups(sizes: Sequence[int], min_count: int) -> Dict[int, np.ndarray]:
@@ -809,12 +873,10 @@ def blue_class_groups(sizes: Sequence[int], min_count: int) -> Dict[int, np.ndar
 
 def _blue_hooks(m: int, inverse: bool):
     """Hooks of a Bluestein class of size m: chirped gather (j < n: x[offs + j] w_j, else
-    0; indices clamped in range), spectrum multiply (bs[b m + r], the spectrum already in
-    the order the forward rows launch produces it, conjugated and reflected for an
-    inverse transform) and the chirped output (k < n: out[offs + k] = c_k w_k scale).
-    Every hook reads and writes at its own position r = io mod m: the forward FFT_m
-    leaves its output in the rows launch's order, the inverse FFT_m starts with rows on
-    that order and ends with columns, whose positions are the natural time order."""
+    0; indices clamped in range) and the chirped output (k < n: out[offs + k] =
+    c_k w_k scale).  Both act at their own position r = io mod m: the first launch
+    reads time order, and the last launch's positions are the natural time order (the
+    spectrum multiply between them is the convolution's kernel map)."""
     from ..shader_map import map as vd_map
     from .hooks import read_op, write_op
 
@@ -838,14 +900,6 @@ def _blue_hooks(m: int, inverse: bool):
             op.register[:] = vc.to_complex(0.0, 0.0)
 
     @vd_map
-    def spec_store(y: vc.Buffer[vc.c64], bs: vc.Buffer[vc.c64]):
-        op = write_op()
-        v = vc.new_register(vc.c64)
-        v[:] = bs[op.io_index]
-        op.register[:] = vc.mult_complex(op.register, v)
-        op.write_to_buffer(y)
-
-    @vd_map
     def chirp_out(out: vc.Buffer[vc.c64], offs: vc.Buffer[vc.i32], nn: vc.Buffer[vc.i32], ch: vc.Buffer[vc.c64],
                   coff: vc.Buffer[vc.i32], scl: vc.Buffer[vc.f32]):
         op = write_op()
@@ -856,7 +910,7 @@ def _blue_hooks(m: int, inverse: bool):
             op.register[:] = vc.mult_complex(op.register, chirp(ch, coff, b, k)) * scl[b]
             op.write_to_buffer(out, io_index=offs[b] + k)
 
-    return gather, spec_store, chirp_out
+    return gather, chirp_out
 
 
 def blue_spectrum_order(m: int) -> np.ndarray:
@@ -870,25 +924,27 @@ def blue_spectrum_order(m: int) -> np.ndarray:
 
 
 def blue_class_kernels(m: int, count: int, inverse: bool = False) -> list:
-    """The vd.fft kernels of a Bluestein class (no device needed)."""
+    """The vd.fft kernels of a Bluestein class (no device needed).  The spectrum goes in
+    as the kernel of vd.fft.convolve's default map X . conj(K), so a class holds
+    K = conj(spectrum)."""
     from .. import dtypes as dt
-    from .shader_factories import make_fft_shader
+    from .shader_factories import make_convolution_shader, make_fft_shader
     c64 = dt.complex64
     fwd = dict(inverse=False, input_type=c64, output_type=c64, compute_type=c64)
     bwd = dict(inverse=True, normalize_inverse=True, input_type=c64, output_type=c64, compute_type=c64)
-    gather, spec_store, chirp_out = _blue_hooks(m, inverse)
+    conv = dict(normalize=True, input_type=c64, output_type=c64, kernel_type=c64, compute_type=c64)
+    gather, chirp_out = _blue_hooks(m, inverse)
     if m not in CLASS_SPLIT:
-        return [make_fft_shader((count, m), 1, input_map=gather, output_map=spec_store, **fwd),
-                make_fft_shader((count, m), 1, output_map=chirp_out, **bwd)]
+        return [make_convolution_shader((count, m), None, 1, 1, input_map=gather, output_map=chirp_out, **conv)]
     n1, n2 = CLASS_SPLIT[m]
     _, _, tw_store, _ = _class_hooks(m, n1, n2)
     shape = (count, n1, n2)
-    # forward: columns over j1 (j = j1 n2 + j2), twiddles, rows over j2 -> C at
-    # r = k1 n2 + k2 (k = k1 + n1 k2); inverse from that order: rows over k2, conjugate
-    # twiddles, columns over k1 -> c at r = j1' n2 + j2' = its natural index
+    # forward columns over j1 (j = j1 n2 + j2) and twiddles; then the rows over j2, whose
+    # output k = k1 + n1 k2 sits at r = k1 n2 + k2, times the spectrum packed in that
+    # order, the inverse rows from that order and the conjugate twiddles; then inverse
+    # columns over k1 -> c at r = j1' n2 + j2' = its natural index
     return [make_fft_shader(shape, 1, input_map=gather, output_map=tw_store, **fwd),
-            make_fft_shader(shape, 2, output_map=spec_store, **fwd),
-            make_fft_shader(shape, 2, output_map=tw_store, **bwd),
+            make_convolution_shader(shape, None, 1, 2, output_map=tw_store, **conv),
             make_fft_shader(shape, 1, output_map=chirp_out, **bwd)]
 
 
@@ -902,6 +958,7 @@ class BlueClass:
         from .. import dtypes as dt
         from ..buffer import Buffer
         self.m, self.count = int(m), len(sizes)
+        self.sizes = np.asarray(sizes, np.int64)
         sizes = np.asarray(sizes, np.int64)
         if self.m not in BLUE_CLASS_LENGTHS or any(bluestein_length(int(n)) != self.m for n in sizes):
             raise ValueError(f"not a Bluestein class of size {m}")
@@ -933,27 +990,33 @@ class BlueClass:
             rt.dtoh(B.ctypes.data, T.chirp_spectrum(n, self.m, rads), self.m * 8, s)
             rt.synchronize(s)
             spec[i] = np.conj(B[order]) if inverse else B[order]
-        self.bs = Buffer((self.count * self.m,), c64)
-        self.bs.write(spec.reshape(-1))
+        # the kernel of the convolution launch: X . conj(K) with K = conj(spectrum)
+        self.ks = Buffer((self.count * self.m,), c64)
+        self.ks.write(np.conj(spec).reshape(-1))
         self.kernels = blue_class_kernels(self.m, self.count, inverse)
-        self.y = Buffer((self.count * self.m,), c64)
-        self.z = self.tw = self.twi = None
+        self.y = self.z = self.tw = self.twi = None
         if self.m in CLASS_SPLIT:
+            self.y = Buffer((self.count * self.m,), c64)
             self.z = Buffer((self.count * self.m,), c64)
-            self.tw = Buffer((self.m,), c64)
-            self.tw.write(split_twiddles(self.m))
-            self.twi = Buffer((self.m,), c64)
-            self.twi.write(split_twiddles(self.m, inverse=True))
+            self.tw = split_twiddle_buffer(self.m)
+            self.twi = split_twiddle_buffer(self.m, inverse=True)
         rt.synchronize()
+
+    def bytes_moved(self) -> int:
+        """Global-memory bytes of the class's launches: the n inputs and outputs and the
+        kernel spectrum, plus two passes over the m-point intermediate per split launch
+        boundary."""
+        n_sum = int(self.sizes.sum())
+        base = 16 * n_sum + 8 * self.count * self.m
+        return base + (32 * self.count * self.m if self.m in CLASS_SPLIT else 0)
 
     def values(self, out_buf, in_buf) -> List[list]:
         io = [self.offs, self.nn, self.ch, self.coff]
-        if self.m not in CLASS_SPLIT:
-            return [[self.y, self.bs, in_buf] + io, [out_buf] + io + [self.scl, self.y]]
+        if self.m not in CLASS_SPLIT:  # output-map args, input-map args, kernel
+            return [[out_buf] + io + [self.scl, in_buf] + io + [self.ks]]
         return [[self.y, self.tw, in_buf] + io,          # chirped columns, twiddles
-                [self.z, self.bs, self.y],               # rows, times the spectrum (rows order)
-                [self.y, self.twi, self.z],              # inverse rows, conjugate twiddles
-                [out_buf] + io + [self.scl, self.y]]     # inverse columns, chirped first n outputs
+                [self.z, self.twi, self.y, self.ks],     # rows * spectrum (rows order) * inverse rows, conj twiddles
+                [out_buf] + io + [self.scl, self.z]]     # inverse columns, chirped first n outputs
 
     def bind(self, out_buf, in_buf):
         packed = []
@@ -976,8 +1039,8 @@ def packed_normal(sizes: Sequence[int], seed: int = 0):
     return parts, np.concatenate(parts).astype(np.complex64)
 
 
-TUNE_TILE_MIN = (4097, 16385, 32769, 65536, TILE_MIN_DIRECT)  # direct lengths tiled from ... / never
-TUNE_KEYS = ("block", "block_first", "max_radix", "tile_min", "classes")  # RaggedFFT arguments chosen by tune()
+TUNE_TILE_MIN = (16385, 65536, TILE_MIN_DIRECT)  # direct lengths tiled from ... / never
+TUNE_KEYS = ("block", "block_first", "max_radix", "tile_min", "classes", "grid")  # RaggedFFT arguments chosen by tune()
 TUNE_TABLE_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_ragged.json")
 
 
@@ -1003,7 +1066,8 @@ def tune(sizes: Sequence[int], buf, reps: int = 5, cache: bool = False, save: bo
          path: str = TUNE_TABLE_PATH) -> Dict:
     """Pick the workgroup sizes (first launch, which holds the whole transforms, and
     the tile launches), the largest radix, the direct length from which transforms
-    are tiled, and whether power-of-two length classes are used, for this batch by
+    are tiled, whether length classes are used and the persistent grid (half a
+    workgroup per CU, leaving CUs to the class launches, or two), for this batch by
     timing each candidate plan (deterministic candidate order; ties keep the earlier
     candidate).  The best entry's keys are RaggedFFT arguments.  cache: return the
     entry stored for (device arch, batch) in the table at ``path`` without timing,
@@ -1015,26 +1079,28 @@ def tune(sizes: Sequence[int], buf, reps: int = 5, cache: bool = False, save: bo
             return {"best": dict(hit), "results": [], "cached": True}
     best = None
     results = []
+    cus = int(rt.ensure_device().compute_units)
     for classes in (CLASS_MIN_COUNT, 0):
-        for block, block_first in ((256, 256), (256, 512), (512, 512)):
+        for block, block_first in ((256, 256), (512, 512)):
             for max_radix in (8, 16):
                 for tile_min in TUNE_TILE_MIN:
-                    plan = RaggedFFT(sizes, block=block, max_radix=max_radix, tile_min=tile_min,
-                                     block_first=block_first, classes=classes)
-                    run = plan.bind(buf)
-                    run()
-                    e0, e1 = rt.Event(), rt.Event()
-                    e0.record()
-                    for _ in range(reps):
+                    for grid in (cus // 2, 2 * cus):
+                        plan = RaggedFFT(sizes, block=block, max_radix=max_radix, tile_min=tile_min,
+                                         block_first=block_first, classes=classes, grid=grid)
+                        run = plan.bind(buf)
                         run()
-                    e1.record()
-                    e1.synchronize()
-                    ms = e0.elapsed_ms(e1) / reps
-                    results.append({"block": block, "block_first": block_first, "max_radix": max_radix,
-                                    "tile_min": tile_min, "classes": classes, "ms": ms})
-                    if best is None or ms < best["ms"]:
-                        best = results[-1]
-                    del run, plan
+                        e0, e1 = rt.Event(), rt.Event()
+                        e0.record()
+                        for _ in range(reps):
+                            run()
+                        e1.record()
+                        e1.synchronize()
+                        ms = e0.elapsed_ms(e1) / reps
+                        results.append({"block": block, "block_first": block_first, "max_radix": max_radix,
+                                        "tile_min": tile_min, "classes": classes, "grid": grid, "ms": ms})
+                        if best is None or ms < best["ms"]:
+                            best = results[-1]
+                        del run, plan
     if save:
         table = load_tune_table(path)
         table[key] = {k: best[k] for k in TUNE_KEYS}

@@ -220,22 +220,26 @@ def current_stream() -> int:
     return _default_stream
 
 
-_side_stream = None
+_side_streams: list = []
+
+
+def side_streams(count: int) -> list:
+    """``count`` further library streams (default priority), created once per process.
+    A plan whose work splits into independent parts forks the parts onto them and
+    joins back with events (stream_wait), inside a graph capture too."""
+    if len(_side_streams) < count:
+        ensure_device()
+        with _device_lock:
+            while len(_side_streams) < count:
+                s = ctypes.c_uint64(0)
+                check(lib().vkx_stream_create(ctypes.byref(s)), "vkx_stream_create")
+                _side_streams.append(s.value)
+    return list(_side_streams[:count])
 
 
 def side_stream() -> int:
-    """A second library stream (default priority), created once per process.  A plan
-    whose work splits into independent parts forks one part onto it and joins back with
-    events (stream_wait), inside a graph capture too."""
-    global _side_stream
-    if _side_stream is None:
-        ensure_device()
-        with _device_lock:
-            if _side_stream is None:
-                s = ctypes.c_uint64(0)
-                check(lib().vkx_stream_create(ctypes.byref(s)), "vkx_stream_create")
-                _side_stream = s.value
-    return _side_stream
+    """The first side stream (see side_streams)."""
+    return side_streams(1)[0]
 
 
 def stream_wait(stream: int, event: "Event") -> None:
