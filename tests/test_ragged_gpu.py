@@ -188,6 +188,41 @@ def test_ragged_bluestein_classes_forward_inverse_and_graph():
     assert O.rel_l2(dst.read(0), packed) < 1e-5
 
 
+@pytest.mark.parametrize("class_streams,persistent_first", [(1, False), (2, True), (3, False)])
+def test_ragged_class_schedules(class_streams, persistent_first):
+    """The same batch on one, two or three side streams, with the persistent launch
+    recorded before or after the classes: short power-of-two classes (64..4096),
+    small-m Bluestein classes (m = 256 and 4096) and persistent-kernel items, forward
+    out of place, graph replays and inverse in place, every transform against FP64."""
+    rng = np.random.default_rng(41 + class_streams)
+    blue = _primes_between(100, 127, 5, rng) * 4 + _primes_between(1030, 2047, 16, rng)  # m = 256, 4096
+    pow2 = [64] * 20 + [512] * 17 + [4096] * 16 + [1024] * 5  # 1024: too few for a class
+    smooth = [100, 4099, 60060, 12288, 97, 3 ** 9, 5 ** 5]
+    sizes = [int(n) for n in rng.permutation(blue + pow2 + smooth * 2)]
+    parts, packed = _packed(sizes, 41)
+    src = vd.asbuffer(packed)
+    dst = vd.Buffer((packed.size,), vd.complex64)
+    plan = R.RaggedFFT(sizes, class_streams=class_streams, persistent_first=persistent_first)
+    kinds = sorted((type(c).__name__, getattr(c, "L", getattr(c, "m", 0))) for c in plan._classes)
+    print(f"classes {kinds}, {plan.n_items} persistent-kernel items", flush=True)
+    assert kinds == [("BlueClass", 256), ("BlueClass", 4096), ("LengthClass", 64), ("LengthClass", 512),
+                     ("LengthClass", 4096)]
+    plan(dst, src)
+    assert not plan.aborted()
+    _check(sizes, parts, dst.read(0))
+    g = vd.CommandGraph()
+    g.record_callable(plan.bind(dst, src))
+    for _ in range(2):
+        dst.write(np.zeros_like(packed))
+        g.submit()
+        vd.synchronize()
+        _check(sizes, parts, dst.read(0))
+    inv = R.RaggedFFT(sizes, inverse=True, class_streams=class_streams, persistent_first=persistent_first)
+    inv(dst)
+    assert not inv.aborted()
+    assert O.rel_l2(dst.read(0), packed) < 1e-5
+
+
 def test_uniform_long_batch_runs_as_a_length_class():
     """vd.fft.fft on (32, 65536): the length is beyond the single-workgroup kernels, the
     batch is one length class."""

@@ -332,3 +332,24 @@ def test_bluestein_class_kernels_compile(m):
     for k in R.blue_class_kernels(m, 17) + R.blue_class_kernels(m, 17, inverse=True):
         assert K.compile_source(k.source)[:4] == b"\x7fELF"
         assert k.code_object_info()["private_bytes"] == 0, k.name
+
+
+def test_class_lanes_balance_by_bytes():
+    """Classes go largest first to the least-loaded lane (ties: the lower lane); every
+    class lands in exactly one lane, at most ``streams`` lanes, none empty."""
+    class Fake:
+        def __init__(self, b):
+            self.b = b
+
+        def bytes_moved(self):
+            return self.b
+
+    cs = [Fake(b) for b in (10, 70, 30, 30, 5, 40)]
+    lanes = R.class_lanes(cs, 2)
+    assert sorted(i for lane in lanes for i in lane) == list(range(len(cs)))
+    # 70 -> lane 0, 40 -> lane 1, 30 -> lane 1 (70 = 70), 30 -> lane 0 (tie), 10 and 5 -> lane 1
+    assert lanes == [[1, 3], [5, 2, 0, 4]]
+    assert [sum(cs[i].b for i in lane) for lane in lanes] == [100, 85]
+    assert R.class_lanes(cs[:1], 3) == [[0]]  # empty lanes are dropped
+    assert R.class_lanes([], 2) == []
+    assert len(R.class_lanes(cs, 8)) == len(cs)
