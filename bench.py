@@ -255,11 +255,58 @@ def bench_conv(steps, warmup, args):
                    "batch_rel_l2_vs_fp64": O.rel_l2(res, ref), "bound": 1e-5},
         "kernels": per_kernel,
     }
+    vendor = vendor_conv(x, K, steps, warmup)
+    if vendor is not None:
+        out["rocfft_torch"] = vendor
     if not args.no_cpu:
         base, single = cpu_baseline_conv(x, K)
         out["cpu_baseline"] = base
         out["cpu_numpy_single_thread"] = single
         out["cpu_host"] = _cpu_info()
+    return out
+
+
+def _torch_timed(fn, inputs, warmup, warm_inputs=None):
+    """Mean ms per call of fn(inp) over ``inputs`` after ``warmup`` calls on
+    ``warm_inputs`` (default: ``inputs``), timed with torch events on torch's current
+    stream."""
+    import torch
+    warm_inputs = warm_inputs or inputs
+    for i in range(warmup):
+        fn(warm_inputs[i % len(warm_inputs)])
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for inp in inputs:
+        fn(inp)
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / len(inputs)
+
+
+def vendor_conv(x, K, steps, warmup):
+    """The same step with torch.fft (rocFFT on ROCm), each step on its own device-resident
+    copy of the input: a vendor reference beside the CPU baseline, not the baseline."""
+    try:
+        import torch
+        if not torch.cuda.is_available():
+            return None
+    except ImportError:
+        return None
+    n = x.shape[-1]
+    xs = [torch.from_numpy(x).cuda() for _ in range(warmup + steps)]
+    kc = torch.from_numpy(np.conj(K).astype(np.complex64)).cuda()
+    ms = _torch_timed(lambda xi: torch.fft.irfft2(torch.fft.rfft2(xi) * kc, s=(n, n)), xs[warmup:], warmup,
+                      xs[:warmup])
+    y = torch.fft.irfft2(torch.fft.rfft2(xs[0]) * kc, s=(n, n)).cpu().numpy()
+    import oracle as O
+    ref = O.convolve2DR(x[:2], K, kernel_inner_only=True)
+    out = {"ms_per_step": round(ms, 5), "GB_s": round((x.nbytes * 2 + K.astype(np.complex64).nbytes) / (ms * 1e-3) / 1e9, 1),
+           "rel_l2_vs_fp64_images_0_1": O.rel_l2(y[:2], ref),
+           "impl": "torch.fft.irfft2(torch.fft.rfft2(x) * conj(K)) (rocFFT), out of place",
+           "note": "vendor reference on the same GPU, not the baseline"}
+    del xs
+    torch.cuda.empty_cache()
     return out
 
 
@@ -314,6 +361,17 @@ def bench_c2c(steps, warmup, args):
            "fwd_kernel_ms": round(ms_fwd, 5), "fwd_kernel_gbs": round(2 * x.nbytes / (ms_fwd * 1e-3) / 1e9, 1),
            "fwd_kernel_frac": round(2 * x.nbytes / (ms_fwd * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
            "parity": {"fwd_rel_l2_vs_fp64": rel_fwd, "roundtrip_rel_l2": O.rel_l2(res, x), "bound": 5e-6}}
+    try:
+        import torch
+        if torch.cuda.is_available():
+            xt = torch.from_numpy(x).cuda()
+            ms_v = _torch_timed(lambda v: torch.fft.ifft(torch.fft.fft(v, dim=-1), dim=-1), [xt] * steps, warmup)
+            out["rocfft_torch"] = {"ms_per_step": round(ms_v, 5), "GB_s": round(alg / (ms_v * 1e-3) / 1e9, 1),
+                                   "impl": "torch.fft.fft then torch.fft.ifft (rocFFT), out of place",
+                                   "note": "vendor reference on the same GPU, not the baseline"}
+            del xt
+    except ImportError:
+        pass
     if not getattr(args, "no_cpu", False):
         import scipy.fft as sf
         w = _workers()
