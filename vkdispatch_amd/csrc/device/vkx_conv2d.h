@@ -25,16 +25,23 @@
 
 namespace vkx {
 
-// 64-lane, one-transform-per-wave configuration of the Stockham machinery
-template <int N_, int NP_> struct WaveCfg {
+// One transform per TT_ threads (64: one wave, exchanges synchronise the wave only;
+// 128: two waves, exchanges are workgroup barriers, half the points per lane)
+template <int N_, int NP_, int TT_ = 64> struct WaveCfg {
   using real = float;
-  static constexpr int N = N_, TT = 64, B = 1, PAD_SHIFT = 5, NP = NP_;
-  static constexpr bool COLFAST = false, WAVE_LOCAL = true;
+  static constexpr int N = N_, TT = TT_, B = 1, PAD_SHIFT = 5, NP = NP_;
+  static constexpr bool COLFAST = false, WAVE_LOCAL = TT_ == 64;
 };
+
+// threads per transform of a convolution config (C::TT, default one wave)
+template <class C, class = void> struct conv_tt_of { static constexpr int value = 64; };
+template <class C> struct conv_tt_of<C, decltype((void)C::TT)> { static constexpr int value = C::TT; };
+template <class C> using ConvWC = WaveCfg<C::W, C::NP, conv_tt_of<C>::value>;
 
 // Register-resident twiddles (loaded once per task, see TwSet) when a lane's set is
 // small: 14 values for 512 = 8.8.8 and 9 for 256 = 4.4.4.4.  The 27 of 1024 =
-// 16.16.4 keep reading the table (registers are better spent on data there).
+// 16.16.4 on one wave, or 19 of 1024 = 8.8.4.4 on two, keep reading the table
+// (registers are better spent on data there).
 template <class WC, class L> constexpr bool conv_twreg() { return list_tw_regs<WC, L>::value <= 16; }
 
 // C: H, W (= N of the wave transforms), PITCH (floats per real row), IMG (floats
@@ -59,22 +66,32 @@ template <class C> __device__ __forceinline__ ConvArgs<C> conv_args(const KParam
 }
 
 template <class C> struct ConvShape {
-  static constexpr int NV = C::W / 64;               // points per lane
-  static constexpr int RPW = C::RA / C::NW;          // rows per wave in a row task
+  static constexpr int TT = conv_tt_of<C>::value;    // threads per transform
+  static constexpr int NV = C::W / TT;               // points per lane
+  static constexpr int NU = C::NW * 64 / TT;         // transform units per workgroup
+  static constexpr int RPW = C::RA / NU;             // rows per unit in a row task
   static constexpr int NT = C::NW * 64;              // threads per workgroup
   static constexpr int CT = 16;                      // columns per tile (128-byte rows)
-  static constexpr int CB = CT / C::NW;              // columns per wave in a column task
+  static constexpr int CB = CT / C::NW;              // columns per wave in a column task (tile layout)
   static constexpr u32 NA = C::H / C::RA;            // row tasks per pair
   static constexpr u32 NB = C::W / CT;               // column tasks (tiles) per pair
   static constexpr u32 KB = 64;                      // kernel-prep block (KB x KB)
   static constexpr u32 NK = (C::H / KB) * (C::W / KB);  // kernel-prep tasks
-  static constexpr int XCH = C::NW * C::NP;          // per-wave exchange regions
+  static constexpr int XCH = NU * C::NP;             // per-unit exchange regions
   static constexpr int TILE = CT * C::NP;            // one column tile, column-major
   static constexpr int LDS_ALL = TILE > XCH ? TILE : XCH;
   static_assert(C::H == C::W, "square images");
-  static_assert(C::W % 64 == 0 && C::RA % C::NW == 0 && CT % C::NW == 0, "tiling");
+  static_assert(TT == 64 || TT == 128, "one or two waves per transform");
+  static_assert(C::W % TT == 0 && C::RA % NU == 0 && CT % C::NW == 0, "tiling");
+  static_assert(TT == 64 || C::LAYOUT == 1, "two-wave transforms take the column-major layout");
   static_assert(C::NP >= C::H + C::H / 32, "LDS region");
 };
+
+// transform unit of a thread and its index inside the unit (wave / lane for one wave)
+template <class C> __device__ __forceinline__ int conv_unit(int tid) {
+  return __builtin_amdgcn_readfirstlane(tid / ConvShape<C>::TT);
+}
+template <class C> __device__ __forceinline__ int conv_lane(int tid) { return tid % ConvShape<C>::TT; }
 
 // element (ky, kx) of a tile-major pair spectrum
 template <class C> __device__ __forceinline__ u32 tile_index(u32 ky, u32 kx) {
@@ -229,37 +246,38 @@ __device__ __forceinline__ void conv_rows_inv(const ConvArgs<C>& a, cpx<float>* 
 // streams whole contiguous columns, each wave on its own CB columns. ----
 template <class C>
 __device__ __forceinline__ void conv_rows_fwd_cm(const ConvArgs<C>& a, cpx<float>* lds, u32 pair, u32 slice,
-                                                 int tid, int wave, int lane) {
+                                                 int tid) {
   using SH = ConvShape<C>;
-  using WC = WaveCfg<C::W, C::NP>;
+  using WC = ConvWC<C>;
   using L = typename C::fwd;
   using F0 = typename pl_first<L>::type;
   using FL = typename pl_last<L>::type;
   constexpr int IT = C::W * C::RA / SH::NT;
   static_assert(C::W * C::RA % SH::NT == 0 && C::TP > C::RA, "transpose tile");
+  const int u = conv_unit<C>(tid), t = conv_lane<C>(tid);
   const u32 ia = 2 * pair, ib = 2 * pair + 1;
   const bool has_b = ib < C::NIMG;
   const u32 y0 = slice * C::RA;
   cpx<float>* S = a.scr + (u64)pair * (u64)(C::H * C::W);
   TwSet<WC, L, conv_twreg<WC, L>()> ts;
-  ts.load(lane, a.twf);  // in flight with the row loads
+  ts.load(t, a.twf);  // in flight with the row loads
   cpx<float> v[SH::RPW][SH::NV];
 #pragma unroll
   for (int i = 0; i < SH::RPW; ++i) {
-    const u32 y = y0 + (u32)(wave + C::NW * i);
+    const u32 y = y0 + (u32)(u + SH::NU * i);
     const float* ra = a.x + (u64)ia * C::IMG + y * C::PITCH;
     const float* rb = a.x + (u64)ib * C::IMG + y * C::PITCH;
-    for_layout<WC, F0::R>(lane, [&](auto slot, u32 n) {
+    for_layout<WC, F0::R>(t, [&](auto slot, u32 n) {
       v[i][decltype(slot)::value] = cpx<float>{ra[n], has_b ? rb[n] : 0.f};
     });
   }
 #pragma unroll
-  for (int i = 0; i < SH::RPW; ++i) ts.template run<-1>(v[i], lds + wave * C::NP, lane, 0);
+  for (int i = 0; i < SH::RPW; ++i) ts.template run<-1>(v[i], lds + u * C::NP, t, 0);
   __syncthreads();  // the transpose tile overlaps the exchange regions
 #pragma unroll
   for (int i = 0; i < SH::RPW; ++i) {
-    const u32 yl = (u32)(wave + C::NW * i);
-    for_layout<WC, FL::R>(lane, [&](auto slot, u32 kx) { lds[kx * C::TP + yl] = v[i][decltype(slot)::value]; });
+    const u32 yl = (u32)(u + SH::NU * i);
+    for_layout<WC, FL::R>(t, [&](auto slot, u32 kx) { lds[kx * C::TP + yl] = v[i][decltype(slot)::value]; });
   }
   __syncthreads();
 #pragma unroll
@@ -269,47 +287,48 @@ __device__ __forceinline__ void conv_rows_fwd_cm(const ConvArgs<C>& a, cpx<float
   }
 }
 
-// Column task: CB columns per wave for PP consecutive pairs; each wave loads its
+// Column task: CB columns per unit for PP consecutive pairs; each unit loads its
 // Kt columns once and reuses them for every pair of the task.
 template <class C>
 __device__ __forceinline__ void conv_cols_cm(const ConvArgs<C>& a, const cpx<float>* kt, cpx<float>* lds, u32 pair0,
-                                             u32 slice, int wave, int lane) {
+                                             u32 slice, int tid) {
   using SH = ConvShape<C>;
-  using WC = WaveCfg<C::W, C::NP>;
+  using WC = ConvWC<C>;
   using L = typename C::fwd;
   using LI = typename C::inv;
   using F0 = typename pl_first<L>::type;
   using FL = typename pl_last<L>::type;
   using IL = typename pl_last<LI>::type;
-  const u32 kx0 = slice * (u32)(C::NW * C::CB) + (u32)(wave * C::CB);
-  cpx<float>* xch = lds + wave * C::NP;
+  const int u = conv_unit<C>(tid), t = conv_lane<C>(tid);
+  const u32 kx0 = slice * (u32)(SH::NU * C::CB) + (u32)(u * C::CB);
+  cpx<float>* xch = lds + u * C::NP;
   // every transform of the task uses the same per-lane twiddles: read them once.  A
   // twiddle table depends only on N and the radix list (the inverse passes conjugate),
   // so when the inverse list equals the forward one both directions share one set.
   constexpr bool SAME = __is_same(L, LI);
   TwSet<WC, L, conv_twreg<WC, L>()> tsf;
   TwSet<WC, LI, SAME ? false : conv_twreg<WC, LI>()> tsi_own;
-  tsf.load(lane, a.twf);
-  tsi_own.load(lane, a.twi);
+  tsf.load(t, a.twf);
+  tsi_own.load(t, a.twi);
 #pragma unroll
   for (int cc = 0; cc < C::CB; ++cc) {
     const u32 kx = kx0 + (u32)cc;
     cpx<float> kv[SH::NV];
-    for_layout<WC, FL::R>(lane, [&](auto slot, u32 ky) { kv[decltype(slot)::value] = kt[kx * C::H + ky]; });
+    for_layout<WC, FL::R>(t, [&](auto slot, u32 ky) { kv[decltype(slot)::value] = kt[kx * C::H + ky]; });
 #pragma nounroll
     for (u32 pp = 0; pp < (u32)C::PP; ++pp) {
       const u32 pair = pair0 + pp;
-      if (pair >= C::NPAIRS) break;
+      if (pair >= C::NPAIRS) break;  // uniform over the workgroup (pair0 is the task's)
       cpx<float>* col = a.scr + (u64)pair * (u64)(C::H * C::W) + kx * C::H;
       cpx<float> v[SH::NV];
-      for_layout<WC, F0::R>(lane, [&](auto slot, u32 n) { v[decltype(slot)::value] = col[n]; });
-      tsf.template run<-1>(v, xch, lane, 0);
+      for_layout<WC, F0::R>(t, [&](auto slot, u32 n) { v[decltype(slot)::value] = col[n]; });
+      tsf.template run<-1>(v, xch, t, 0);
 #pragma unroll
       for (int i = 0; i < SH::NV; ++i) v[i] = cmul(v[i], kv[i]);
       block_sync<WC>();
-      if constexpr (SAME) tsf.template run<1>(v, xch, lane, 0);
-      else tsi_own.template run<1>(v, xch, lane, 0);
-      for_layout<WC, IL::R>(lane, [&](auto slot, u32 n) { col[n] = v[decltype(slot)::value]; });
+      if constexpr (SAME) tsf.template run<1>(v, xch, t, 0);
+      else tsi_own.template run<1>(v, xch, t, 0);
+      for_layout<WC, IL::R>(t, [&](auto slot, u32 n) { col[n] = v[decltype(slot)::value]; });
       block_sync<WC>();
     }
   }
@@ -317,47 +336,48 @@ __device__ __forceinline__ void conv_cols_cm(const ConvArgs<C>& a, const cpx<flo
 
 template <class C>
 __device__ __forceinline__ void conv_rows_inv_cm(const ConvArgs<C>& a, cpx<float>* lds, u32 pair, u32 slice,
-                                                 int tid, int wave, int lane) {
+                                                 int tid) {
   using SH = ConvShape<C>;
-  using WC = WaveCfg<C::W, C::NP>;
+  using WC = ConvWC<C>;
   using LI = typename C::inv;
   using I0 = typename pl_first<LI>::type;
   using IL = typename pl_last<LI>::type;
   constexpr int IT = C::W * C::RA / SH::NT;
+  const int u = conv_unit<C>(tid), t = conv_lane<C>(tid);
   const u32 ia = 2 * pair, ib = 2 * pair + 1;
   const bool has_b = ib < C::NIMG;
   const u32 y0 = slice * C::RA;
   const cpx<float>* S = a.scr + (u64)pair * (u64)(C::H * C::W);
   TwSet<WC, LI, conv_twreg<WC, LI>()> ts;
-  ts.load(lane, a.twi);  // in flight with the tile loads; reused by every row
+  ts.load(t, a.twi);  // in flight with the tile loads; reused by every row
   {
-    cpx<float> t[IT];  // all loads in flight before the first LDS write
+    cpx<float> tl[IT];  // all loads in flight before the first LDS write
 #pragma unroll
     for (int i = 0; i < IT; ++i) {
       const u32 e = (u32)(tid + i * SH::NT), yl = e % C::RA, kx = e / C::RA;
-      t[i] = S[kx * C::H + y0 + yl];
+      tl[i] = S[kx * C::H + y0 + yl];
     }
 #pragma unroll
     for (int i = 0; i < IT; ++i) {
       const u32 e = (u32)(tid + i * SH::NT), yl = e % C::RA, kx = e / C::RA;
-      lds[kx * C::TP + yl] = t[i];
+      lds[kx * C::TP + yl] = tl[i];
     }
   }
   __syncthreads();
   cpx<float> v[SH::RPW][SH::NV];
 #pragma unroll
   for (int i = 0; i < SH::RPW; ++i) {
-    const u32 yl = (u32)(wave + C::NW * i);
-    for_layout<WC, I0::R>(lane, [&](auto slot, u32 kx) { v[i][decltype(slot)::value] = lds[kx * C::TP + yl]; });
+    const u32 yl = (u32)(u + SH::NU * i);
+    for_layout<WC, I0::R>(t, [&](auto slot, u32 kx) { v[i][decltype(slot)::value] = lds[kx * C::TP + yl]; });
   }
   __syncthreads();  // the exchange regions overlap the transpose tile
 #pragma unroll
   for (int i = 0; i < SH::RPW; ++i) {
-    const u32 y = y0 + (u32)(wave + C::NW * i);
-    ts.template run<1>(v[i], lds + wave * C::NP, lane, 0);
+    const u32 y = y0 + (u32)(u + SH::NU * i);
+    ts.template run<1>(v[i], lds + u * C::NP, t, 0);
     float* ra = a.x + (u64)ia * C::IMG + y * C::PITCH;
     float* rb = a.x + (u64)ib * C::IMG + y * C::PITCH;
-    for_layout<WC, IL::R>(lane, [&](auto slot, u32 n) {
+    for_layout<WC, IL::R>(t, [&](auto slot, u32 n) {
       const cpx<float> z = v[i][decltype(slot)::value];
       ra[n] = z.x;
       if (has_b) rb[n] = z.y;
@@ -375,7 +395,7 @@ __device__ __forceinline__ void conv2d_phase_body(const KParams& prm) {
   constexpr bool CM = C::LAYOUT == 1;
   constexpr int ROWS_LDS = CM ? (C::W * C::TP > SH::XCH ? C::W * C::TP : SH::XCH) : SH::XCH;
   constexpr int LDS_ELEMS = PHASE == 1 ? (CM ? SH::XCH : SH::TILE) : ROWS_LDS;
-  constexpr u32 NBT = CM ? (u32)(C::W / (C::NW * C::CB)) : SH::NB;  // column tasks per pair (group)
+  constexpr u32 NBT = CM ? (u32)(C::W / (SH::NU * C::CB)) : SH::NB;  // column tasks per pair (group)
   constexpr u32 NPG = CM ? (C::NPAIRS + C::PP - 1) / C::PP : C::NPAIRS;  // pair groups
   __shared__ cpx<float> lds[LDS_ELEMS];
   const ConvArgs<C> a = conv_args<C>(prm);
@@ -385,19 +405,19 @@ __device__ __forceinline__ void conv2d_phase_body(const KParams& prm) {
   const u32 wg = blockIdx.x;
   if constexpr (PHASE == 0) {
     if (wg < C::NPAIRS * SH::NA) {
-      if constexpr (CM) conv_rows_fwd_cm<C>(a, lds, wg / SH::NA, wg % SH::NA, tid, wave, lane);
+      if constexpr (CM) conv_rows_fwd_cm<C>(a, lds, wg / SH::NA, wg % SH::NA, tid);
       else conv_rows_fwd<C>(a, lds, wg / SH::NA, wg % SH::NA, wave, lane);
     } else if (wg < C::NPAIRS * SH::NA + SH::NK) {
       conv_kprep<C>(a.K, a.kt, wg - C::NPAIRS * SH::NA, tid);
     }
   } else if constexpr (PHASE == 1) {
     if (wg < NPG * NBT) {
-      if constexpr (CM) conv_cols_cm<C>(a, a.kt, lds, (wg % NPG) * C::PP, wg / NPG, wave, lane);
+      if constexpr (CM) conv_cols_cm<C>(a, a.kt, lds, (wg % NPG) * C::PP, wg / NPG, tid);
       else conv_cols<C>(a, a.kt, lds, wg % C::NPAIRS, wg / C::NPAIRS, tid, wave, lane);
     }
   } else {
     if (wg < C::NPAIRS * SH::NA) {
-      if constexpr (CM) conv_rows_inv_cm<C>(a, lds, wg / SH::NA, wg % SH::NA, tid, wave, lane);
+      if constexpr (CM) conv_rows_inv_cm<C>(a, lds, wg / SH::NA, wg % SH::NA, tid);
       else conv_rows_inv<C>(a, lds, wg / SH::NA, wg % SH::NA, wave, lane);
     }
   }

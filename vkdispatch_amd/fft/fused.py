@@ -28,6 +28,8 @@ from .planner import KernelPlan
 from .shader_factories import make_convolution_shader, make_fft_shader, register_plan_cache
 
 _FUSED_RADICES = {256: (4, 4, 4, 4), 512: (8, 8, 8), 1024: (16, 16, 4)}
+# two waves (128 threads) per transform: every pass keeps 8 points per lane (radices <= 8)
+_FUSED_RADICES_TT128 = {1024: (8, 8, 4, 4)}
 
 
 def _fused_enabled() -> bool:
@@ -39,15 +41,19 @@ KPREP_BLOCK = 64   # kernel-prep task: 64 x 64 elements of Kt
 
 
 def _conv_cfg(batch: int, H: int, W: int, normalize: bool, waves: int, rows: int, layout: int = 0,
-              cols_per_wave: int = 1, pairs_per_task: int = 1, extra: Optional[Dict[str, int]] = None
-              ) -> Tuple[str, KernelPlan]:
+              cols_per_wave: int = 1, pairs_per_task: int = 1, extra: Optional[Dict[str, int]] = None,
+              tt: int = 64) -> Tuple[str, KernelPlan]:
     """``struct Cfg`` of the phase kernels and the matching KernelPlan
-    (radices, twiddle tables)."""
+    (radices, twiddle tables).  ``tt``: threads per transform (64: one wave; 128: two
+    waves, column-major layout only)."""
     if H != W or W not in _FUSED_RADICES:
         raise ValueError("the pair-packed 2-D convolution takes square 256, 512 or 1024 images")
-    if waves not in (1, 2, 4, 8, 16) or rows % waves or H % rows:
+    units = waves * 64 // tt
+    if tt not in (64, 128) or (tt == 128 and (layout != 1 or W not in _FUSED_RADICES_TT128)):
+        raise ValueError("two-wave transforms take 1024-point images and the column-major layout")
+    if waves not in (1, 2, 4, 8, 16) or units < 1 or rows % units or H % rows:
         raise ValueError("tiling does not divide the image")
-    radices = _FUSED_RADICES[W]
+    radices = _FUSED_RADICES_TT128[W] if tt == 128 else _FUSED_RADICES[W]
     NP = W + W // 32 + 1  # odd stride between LDS column regions: conflict-free tile transposes
     pitch = 2 * (W // 2 + 1)
     npairs = (batch + 1) // 2
@@ -63,6 +69,7 @@ def _conv_cfg(batch: int, H: int, W: int, normalize: bool, waves: int, rows: int
         f"  static constexpr vkx::u32 PITCH = {pitch}u, KW = {W // 2 + 1}u, NIMG = {batch}u, NPAIRS = {npairs}u;",
         f"  static constexpr vkx::u64 IMG = {H * pitch}ull;",
         f"  static constexpr double SCALE = {scale!r};",
+        *([f"  static constexpr int TT = {tt};"] if tt != 64 else []),
         *[f"  static constexpr int {k} = {v};" for k, v in (extra or {}).items()],
         f"  using fwd = {pl(plan.fwd)};",
         f"  using inv = {pl(plan.inv)};",
@@ -94,7 +101,7 @@ class Conv2DPhases:
     def __init__(self, batch: int, H: int, W: int, normalize: bool = True, waves: int = 8, rows: int = 16,
                  col_waves: int = 8, layout: str = "column", cols_per_wave: int = 1, pairs_per_task: int = 4,
                  waves_per_simd: Tuple[Optional[int], Optional[int], Optional[int]] = (None, None, None),
-                 irow_waves: Optional[int] = None):
+                 irow_waves: Optional[int] = None, tt=64, irow_rows: Optional[int] = None):
         self.batch, self.H, self.W = int(batch), int(H), int(W)
         self.npairs = (self.batch + 1) // 2
         self.layout = layout
@@ -102,15 +109,22 @@ class Conv2DPhases:
         self.kernels_ = []
         na = H // rows
         pp = pairs_per_task if lay == 1 else 1
-        nb = W // COL_TILE if lay == 0 else W // (col_waves * cols_per_wave)
+        tts = tuple(tt) if isinstance(tt, (tuple, list)) else (tt, tt, tt)  # threads per transform per phase
+        if len(tts) != 3:
+            raise ValueError("tt is one value or one per phase")
+        nb = W // COL_TILE if lay == 0 else W // (col_waves * 64 // tts[1] * cols_per_wave)
         npg = (self.npairs + pp - 1) // pp
         nk = (H // KPREP_BLOCK) * (W // KPREP_BLOCK)
         for phase, tag in enumerate(("rows", "cols", "irows")):
             nw = col_waves if phase == 1 else (irow_waves or waves) if phase == 2 else waves
-            cfg, plan = _conv_cfg(batch, H, W, normalize, nw, rows if phase != 1 else 16, lay, cols_per_wave, pp)
-            nwg = (self.npairs * na + nk, npg * nb, self.npairs * na)[phase]
+            nrows = (rows, 16, irow_rows or rows)[phase]
+            if H % nrows:
+                raise ValueError("tiling does not divide the image")
+            cfg, plan = _conv_cfg(batch, H, W, normalize, nw, nrows, lay, cols_per_wave, pp, tt=tts[phase])
+            nwg = (self.npairs * na + nk, npg * nb, self.npairs * (H // nrows))[phase]
             plan = KernelPlan(**{**plan.__dict__, "NWG": nwg})
-            symbol = kernel_symbol("conv2d", plan, f"pairs_{tag}" + ("_cm" if lay else ""))
+            symbol = kernel_symbol("conv2d", plan, f"pairs_{tag}" + ("_cm" if lay else "") +
+                                   ("_t128" if tts[phase] == 128 else ""))
             src = _conv_tu(cfg, symbol, 64 * nw, waves_per_simd[phase],
                            f"vkx::conv2d_phase_body<Cfg, {phase}>(prm);")
             args = [Arg("output", "buffer", dt.complex64, 0), Arg("kernel", "buffer", dt.complex64, 1),
@@ -152,8 +166,13 @@ class Conv2DPhases:
 
 
 # phase-kernel tilings: for 512 the row kernels run one row per wave (16 waves, 64 VGPRs,
-# no spills), so two workgroups per CU hold 8 waves per SIMD while they read HBM
-_PHASE_TILING = {512: dict(waves=16, irow_waves=8)}
+# no spills), so two workgroups per CU hold 8 waves per SIMD while they read HBM.  For
+# 1024 the forward rows and the columns run each transform on two waves (8 points per
+# lane: 88 / 123 VGPRs instead of 153 / 176; columns 144.8 -> 73.9 us, rows 71.3 ->
+# 66.1 us on 32 images), the inverse rows stay on one wave (59.0 vs 68.6 us).
+_PHASE_TILING = {512: dict(waves=16, irow_waves=8),
+                 1024: dict(waves=16, rows=8, tt=(128, 128, 64), col_waves=4, cols_per_wave=1, pairs_per_task=4,
+                            irow_waves=8, irow_rows=16)}
 
 
 @lru_cache(maxsize=32)
