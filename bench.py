@@ -476,7 +476,7 @@ def bench_mixed3d(steps, warmup, args):
         for _ in range(warmup):
             run()
     vd.synchronize()
-    rounds = 7
+    rounds = 15
     samples = {k: [] for k in legs}
     for _ in range(rounds):
         for k, run in legs.items():
@@ -662,7 +662,7 @@ def main():
                 "mfma_utilisation": d["bf16"]["mfma_utilisation"],
                 "fp16_storage_speedup": d["fp16"]["speedup_vs_fp32"],
                 "fp16_rel_l2_vs_fp64_quantized": d["fp16"]["rel_l2_vs_fp64_quantized"],
-                "timing": "median of 7 interleaved rounds of back-to-back launches per leg"}
+                "timing": "median of 15 interleaved rounds of back-to-back launches per leg"}
             if "cpu_pocketfft" in m:
                 out["bf16_mfma_vs_fp32"]["cpu_pocketfft"] = m["cpu_pocketfft"]
             # the other workloads of the path, each with its own parity and CPU baseline
