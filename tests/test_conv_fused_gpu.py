@@ -48,7 +48,7 @@ def _run(x, K, kernel_inner_only=True, normalize=True):
     return buf.read_real(0)
 
 
-@pytest.mark.parametrize("batch,n", [(8, 512), (5, 256), (2, 1024), (1, 512), (17, 512), (40, 256)])
+@pytest.mark.parametrize("batch,n", [(8, 512), (5, 256), (2, 1024), (3, 1024), (1, 512), (17, 512), (40, 256)])
 def test_fused_matches_oracle(batch, n):
     print(f"pair-packed convolve2DR batch={batch} n={n}", flush=True)
     rng = np.random.default_rng(batch * 1000 + n)

@@ -248,7 +248,7 @@ def composed_kernels(real_shape, kernel_inner_only=True, compute_type=dt.complex
 
 _STANDARD_FUSED = ((64, 512, True), (8, 512, True), (16, 512, True), (6, 512, True), (4, 512, True),
                    (1, 512, True), (17, 512, True), (5, 256, True), (4, 256, True), (3, 256, True),
-                   (3, 256, False), (40, 256, True), (2, 1024, True))
+                   (3, 256, False), (40, 256, True), (2, 1024, True), (3, 1024, True), (32, 1024, True))
 
 
 def standard_kernels() -> List:
