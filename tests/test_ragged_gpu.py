@@ -262,7 +262,7 @@ def test_tune_is_cached_per_batch(tmp_path):
     path = str(tmp_path / "tuned.json")
     first = R.tune(sizes, buf, reps=1, cache=True, save=True, path=path)
     # classes on / off x two block sizes x two largest radices x tile thresholds x two grids
-    assert not first["cached"] and len(first["results"]) == 2 * 2 * 2 * len(R.TUNE_TILE_MIN) * 2
+    assert not first["cached"] and len(first["results"]) == 2 * 2 * 2 * len(R.TUNE_TILE_MIN) * 3
     again = R.tune(sizes, buf, reps=1, cache=True, path=path)
     assert again["cached"] and {k: again["best"][k] for k in R.TUNE_KEYS} == {k: first["best"][k] for k in R.TUNE_KEYS}
 

@@ -5,9 +5,11 @@
 // (the host orders the queue); every workgroup leaves the loop when the counter
 // passes the item count.  Stockham radices (2..13, 16) are dispatched at run time
 // to the same compile-time butterflies the specialised kernels use (vkx_fft.h).
-//   * whole-transform items (lengths <= 4096 ping-pong between two LDS buffers;
-//     longer lengths without a split and Bluestein lengths whose power-of-two
-//     size has no split use this workgroup's slice of a global scratch arena);
+//   * whole-transform items (lengths up to one LDS buffer -- 4224 points, 10232 in
+//     workgroups of 512 threads -- ping-pong between two LDS buffers; longer lengths
+//     without a split alternate between the output and this workgroup's slice of a
+//     global scratch arena, and Bluestein lengths whose power-of-two size has no
+//     split run in that slice);
 //   * longer lengths, split L = n1 * n2 by the host: column-tile and row-tile
 //     items of the four-step algorithm, spread over all workgroups, so one long
 //     transform no longer runs on one workgroup.  The host either launches the
@@ -43,8 +45,11 @@ struct RTask {
 };
 static_assert(sizeof(RTask) == 128, "RTask layout");
 
-constexpr int kLdsPingPong = 4096;  // complex64 elements of one contiguous LDS batch
-constexpr int kLdsBuf = 4096 + 128;  // one LDS buffer: room for padded tiles
+// One LDS buffer (complex64 elements; a workgroup has two): room for a padded tile of
+// 4096 points.  Workgroups of 512 threads, one per CU at this kernel's register use,
+// take the whole 160 KB instead, so whole transforms up to kLdsBig points stay in LDS.
+constexpr int kLdsBuf = 4096 + 128;
+constexpr int kLdsBig = 10232;  // 2 x 10232 x 8 B + the queue slot fit in 160 KB
 
 template <int R, int DIR>
 __device__ __forceinline__ void bfly(const c64* __restrict__ src, c64* __restrict__ dst, const c64* __restrict__ tw,
@@ -95,6 +100,67 @@ __device__ void pass_dispatch(int R, const c64* src, c64* dst, const c64* tw, in
   }
 }
 
+// One pass of a single length-n transform from global to global memory whose output
+// runs (ns consecutive elements) are shorter than a cache line: each round takes whole
+// groups of ns butterflies, so its R outputs per butterfly fill one contiguous range,
+// which goes through LDS (`stage`, padded one element per 16) and out in full lines.
+// Every thread runs the same rounds (the barriers are uniform).
+template <int R, int DIR>
+__device__ __forceinline__ void bfly_staged(const c64* __restrict__ src, c64* __restrict__ dst,
+                                            const c64* __restrict__ tw, int n, int ns, float scale, c64* stage) {
+  const int W = n / R;  // a multiple of ns
+  const int C = (int)(blockDim.x / ns) * ns;
+  for (int j0 = 0; j0 < W; j0 += C) {
+    const int j = j0 + (int)threadIdx.x;
+    if ((int)threadIdx.x < C && j < W) {
+      const int k = j % ns;
+      c64 v[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[r] = src[j + r * W];
+      if (ns > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          c64 w = tw[(ns - 1) + (r - 1) * ns + k];
+          if (DIR > 0) w.y = -w.y;
+          v[r] = vkx::cmul(v[r], w);
+        }
+      }
+      vkx::DFT<R, DIR>::run(v);
+      const int q0 = (j - j0 - k) * R + k;  // output position relative to j0 R
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int q = q0 + r * ns;
+        stage[q + (q >> 4)] = v[r] * scale;
+      }
+    }
+    __syncthreads();
+    const int cnt = min(C, W - j0) * R;
+    for (int i = threadIdx.x; i < cnt; i += blockDim.x) dst[j0 * R + i] = stage[i + (i >> 4)];
+    __syncthreads();
+  }
+}
+
+template <int DIR>
+__device__ void pass_dispatch_staged(int R, const c64* src, c64* dst, const c64* tw, int n, int ns, float scale,
+                                     c64* stage) {
+  switch (R) {
+    case 2: bfly_staged<2, DIR>(src, dst, tw, n, ns, scale, stage); break;
+    case 3: bfly_staged<3, DIR>(src, dst, tw, n, ns, scale, stage); break;
+    case 4: bfly_staged<4, DIR>(src, dst, tw, n, ns, scale, stage); break;
+    case 5: bfly_staged<5, DIR>(src, dst, tw, n, ns, scale, stage); break;
+    case 6: bfly_staged<6, DIR>(src, dst, tw, n, ns, scale, stage); break;
+    case 7: bfly_staged<7, DIR>(src, dst, tw, n, ns, scale, stage); break;
+    case 8: bfly_staged<8, DIR>(src, dst, tw, n, ns, scale, stage); break;
+    case 9: bfly_staged<9, DIR>(src, dst, tw, n, ns, scale, stage); break;
+    case 10: bfly_staged<10, DIR>(src, dst, tw, n, ns, scale, stage); break;
+    case 11: bfly_staged<11, DIR>(src, dst, tw, n, ns, scale, stage); break;
+    case 12: bfly_staged<12, DIR>(src, dst, tw, n, ns, scale, stage); break;
+    case 13: bfly_staged<13, DIR>(src, dst, tw, n, ns, scale, stage); break;
+    case 16: bfly_staged<16, DIR>(src, dst, tw, n, ns, scale, stage); break;
+    default: break;  // the host planner emits only the radices above
+  }
+}
+
 __device__ void copy_elems(const c64* src, c64* dst, int total) {
   for (int i = threadIdx.x; i < total; i += blockDim.x) dst[i] = src[i];
 }
@@ -123,6 +189,35 @@ __device__ void run_fft(const c64* src, c64* dst, c64* T0, c64* T1, const RTask&
   }
 }
 
+// The same pass list for one transform (count 1) through global memory: the passes
+// alternate between T0 and dst, so that the last one lands in dst and a transform
+// touches one scratch buffer of its length instead of two.  In place (src == dst), a
+// first pass that would write dst goes to T1.  Passes whose output runs are shorter
+// than 16 elements (128 bytes) are written through `stage` (LDS, room for
+// blockDim.x * 17 elements).  Needs npass >= 2.  Ends with a barrier.
+__device__ void run_fft_global(const c64* src, c64* dst, c64* T0, c64* T1, c64* stage, const RTask& t, int n,
+                               const int* radix, int npass, int dir, float scale) {
+  const c64* tw = reinterpret_cast<const c64*>(t.tw);
+  int ns = 1;
+  const c64* in = src;
+  for (int p = 0; p < npass; ++p) {
+    const bool last = p == npass - 1;
+    c64* out = ((npass - 1 - p) & 1) ? T0 : dst;
+    if (p == 0 && out == src) out = T1;
+    const float sc = last ? scale : 1.0f;
+    if (ns < 16) {
+      if (dir > 0) pass_dispatch_staged<1>(radix[p], in, out, tw, n, ns, sc, stage);
+      else pass_dispatch_staged<-1>(radix[p], in, out, tw, n, ns, sc, stage);
+    } else {
+      if (dir > 0) pass_dispatch<1>(radix[p], in, out, tw, n, 1, ns, sc, n);
+      else pass_dispatch<-1>(radix[p], in, out, tw, n, 1, ns, sc, n);
+    }
+    __syncthreads();
+    ns *= radix[p];
+    in = out;
+  }
+}
+
 // exp(-2 pi i p / L) for 0 <= p < 2^31 (the host keeps tiled lengths <= 2^18)
 __device__ __forceinline__ c64 twiddle_L(int p, int L) {
   const int q = p % L;
@@ -141,7 +236,10 @@ __device__ __forceinline__ c64 twiddle_L(int p, int L) {
 // pitch n + 1, so a tile crosses HBM once each way.  Threads take consecutive columns
 // (rows) where that makes the global access a run of the tile's width.
 
+// Store functors with kRun take a butterfly's R outputs at once (run<R>), the others
+// one element at a time (st).
 struct LdsIO {  // element (c, j) of an LDS tile
+  static constexpr bool kRun = false;
   c64* p;
   int pitch;
   __device__ __forceinline__ c64 ld(int c, int j) const { return p[c * pitch + j]; }
@@ -167,11 +265,22 @@ struct ColLoad {  // element j1 of column c0 + c: x[j1 n2 + c0 + c]
 };
 
 struct ColStore {  // y[k1 n2 + j2] = v W_L^(k1 j2), j2 = c0 + c
+  static constexpr bool kRun = true;
   c64* dst;
   int n2, c0, L;
-  __device__ __forceinline__ void st(int c, int k1, c64 v) const {
+  // outputs k1 = base + r ns of one butterfly (k1 j2 < n1 n2 = L): the twiddle is
+  // evaluated for every fourth r and stepped by W_L^(ns j2) in between
+  template <int R>
+  __device__ __forceinline__ void run(int c, int base, int ns, const c64 (&v)[R]) const {
     const int j2 = c0 + c;
-    dst[k1 * n2 + j2] = vkx::cmul(v, twiddle_L(k1 * j2, L));
+    const c64 step = R > 1 ? twiddle_L(ns * j2, L) : c64{1.f, 0.f};
+    c64 w{1.f, 0.f};
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int k1 = base + r * ns;
+      w = (r & 3) == 0 ? twiddle_L(k1 * j2, L) : vkx::cmul(w, step);
+      dst[k1 * n2 + j2] = vkx::cmul(v[r], w);
+    }
   }
 };
 
@@ -182,6 +291,7 @@ struct RowLoad {  // element j2 of row r0 + t: y[(r0 + t) n2 + j2]
 };
 
 struct RowStore {  // X[k] with k = r0 + t + n1 k2
+  static constexpr bool kRun = false;
   c64* dst;
   const c64* aux;  // mode 2: Bluestein kernel spectrum; mode 3: chirp
   int n1, r0, m, nvalid, mode, inv;
@@ -235,8 +345,12 @@ __device__ __forceinline__ void bfly_io(const c64* __restrict__ tw, int W, int c
     }
     vkx::DFT<R, -1>::run(v);
     const int base = (j / ns) * ns * R + k;
+    if constexpr (St::kRun) {
+      st.template run<R>(c, base, ns, v);
+    } else {
 #pragma unroll
-    for (int r = 0; r < R; ++r) st.st(c, base + r * ns, v[r]);
+      for (int r = 0; r < R; ++r) st.st(c, base + r * ns, v[r]);
+    }
   }
 }
 
@@ -313,17 +427,20 @@ __device__ void run_tile(const RTask& t, int kind, c64* lds0, c64* lds1) {
   }
 }
 
-__device__ void run_task(const RTask& t, c64* lds0, c64* lds1, c64* scr) {
+// lds_cap: elements of each of the two LDS buffers
+__device__ void run_task(const RTask& t, c64* lds0, c64* lds1, int lds_cap, c64* scr) {
   const c64* in = reinterpret_cast<const c64*>(t.in);
   c64* out = reinterpret_cast<c64*>(t.out);
   const int dir = t.inverse ? 1 : -1;
   const int npass = t.npass & 0xff;
   if (t.m == 0) {
     const int total = t.n * t.count;
-    if (total <= kLdsPingPong) {
+    if (total <= lds_cap) {
       run_fft(in, out, lds0, lds1, t, t.n, t.radix, npass, dir, t.scale);
-    } else {
+    } else if (npass < 2 || t.count != 1) {
       run_fft(in, out, scr, scr + total, t, t.n, t.radix, npass, dir, t.scale);
+    } else {
+      run_fft_global(in, out, scr, scr + total, lds0, t, t.n, t.radix, npass, dir, t.scale);
     }
     return;
   }
@@ -391,9 +508,11 @@ __device__ void wait_counter(int* ctr, int idx, int target, int spin_limit) {
 // ctr[0]: queue position, ctr[1]: abort flag, ctr[2 ...]: per-transform done counters.
 // Items are fetched in queue order and an item only waits for items queued before it,
 // so each wait is for work already held by a running workgroup.
+// CAP: elements of each of the workgroup's two LDS buffers.
+template <int CAP>
 __global__ void __launch_bounds__(512) ragged_kernel(const RTask* __restrict__ tasks, int ntasks, int* ctr,
                                                      c64* scratch, long long scratch_per_wg, int spin_limit) {
-  __shared__ c64 lds[2 * kLdsBuf];
+  __shared__ c64 lds[2 * CAP];
   __shared__ int task_id;
   c64* scr = scratch + (long long)blockIdx.x * scratch_per_wg;
   for (;;) {
@@ -405,7 +524,7 @@ __global__ void __launch_bounds__(512) ragged_kernel(const RTask* __restrict__ t
     const RTask t = tasks[id];
     const int kind = (t.npass >> 16) & 0xf;
     if (kind == kWhole) {
-      run_task(t, lds, lds + kLdsBuf, scr);
+      run_task(t, lds, lds + CAP, CAP, scr);
       continue;
     }
     if (t.dep >= 0) {
@@ -413,7 +532,7 @@ __global__ void __launch_bounds__(512) ragged_kernel(const RTask* __restrict__ t
       __syncthreads();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
-    run_tile(t, kind, lds, lds + kLdsBuf);
+    run_tile(t, kind, lds, lds + CAP);
     if (t.done >= 0) {
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
       __syncthreads();
@@ -454,8 +573,17 @@ int vkx_ragged_exec(uint64_t tasks, int ntasks, uint64_t counters, int ncounters
   if (spin_limit <= 0) spin_limit = 1 << 20;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   hipLaunchKernelGGL(ragged_reset_kernel, dim3(1), dim3(256), 0, s, reinterpret_cast<int*>(counters), ncounters);
-  hipLaunchKernelGGL(ragged_kernel, dim3(grid), dim3(block), 0, s, reinterpret_cast<const RTask*>(tasks), ntasks,
-                     reinterpret_cast<int*>(counters), reinterpret_cast<c64*>(scratch), scratch_per_wg, spin_limit);
+  // workgroups of 512 threads take the large LDS buffers (whole transforms up to kLdsBig
+  // points in LDS); the host sizes the scratch for every whole transform above 4096
+  // points either way
+  if (block == 512)
+    hipLaunchKernelGGL(ragged_kernel<kLdsBig>, dim3(grid), dim3(block), 0, s, reinterpret_cast<const RTask*>(tasks),
+                       ntasks, reinterpret_cast<int*>(counters), reinterpret_cast<c64*>(scratch), scratch_per_wg,
+                       spin_limit);
+  else
+    hipLaunchKernelGGL(ragged_kernel<kLdsBuf>, dim3(grid), dim3(block), 0, s, reinterpret_cast<const RTask*>(tasks),
+                       ntasks, reinterpret_cast<int*>(counters), reinterpret_cast<c64*>(scratch), scratch_per_wg,
+                       spin_limit);
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : vkx_set_hip_error("vkx_ragged_exec: launch", e);
 }

@@ -1067,7 +1067,7 @@ def tune(sizes: Sequence[int], buf, reps: int = 5, cache: bool = False, save: bo
     """Pick the workgroup sizes (first launch, which holds the whole transforms, and
     the tile launches), the largest radix, the direct length from which transforms
     are tiled, whether length classes are used and the persistent grid (half a
-    workgroup per CU, leaving CUs to the class launches, or two), for this batch by
+    workgroup per CU, leaving CUs to the class launches, one or two), for this batch by
     timing each candidate plan (deterministic candidate order; ties keep the earlier
     candidate).  The best entry's keys are RaggedFFT arguments.  cache: return the
     entry stored for (device arch, batch) in the table at ``path`` without timing,
@@ -1084,7 +1084,7 @@ def tune(sizes: Sequence[int], buf, reps: int = 5, cache: bool = False, save: bo
         for block, block_first in ((256, 256), (512, 512)):
             for max_radix in (8, 16):
                 for tile_min in TUNE_TILE_MIN:
-                    for grid in (cus // 2, 2 * cus):
+                    for grid in (cus // 2, cus, 2 * cus):
                         plan = RaggedFFT(sizes, block=block, max_radix=max_radix, tile_min=tile_min,
                                          block_first=block_first, classes=classes, grid=grid)
                         run = plan.bind(buf)
