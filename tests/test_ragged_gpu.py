@@ -81,17 +81,38 @@ def test_ragged_out_of_place_and_graph():
         _check(sizes, parts, dst.read(0))
 
 
-def test_ragged_4096_mixed_sizes_every_transform():
+@pytest.mark.parametrize("block", [256, 512])
+def test_ragged_4096_mixed_sizes_every_transform(block):
     """4096 transforms, lengths 64..65536 (60% pow2, 30% 7-smooth, 10% primes via
-    Bluestein), N(0,1) data from seed 0: every transform within rel-L2 1e-5 of FP64."""
+    Bluestein), N(0,1) data from seed 0: every transform within rel-L2 1e-5 of FP64,
+    with the persistent kernel's 256- and 512-thread forms (the latter keeps whole
+    transforms up to 10232 points in LDS)."""
     sizes = ragged_sizes(4096, seed=0)
     parts, packed = _packed(sizes, 0)
     buf = vd.asbuffer(packed)
-    plan = R.RaggedFFT(sizes)
+    plan = R.RaggedFFT(sizes, block=block)
     plan(buf)
     assert not plan.aborted()
     worst = _check(sizes, parts, buf.read(0))
     assert worst < 1e-5
+
+
+@pytest.mark.parametrize("block", [256, 320, 512])
+def test_ragged_whole_transforms_lds_and_global(block):
+    """Whole transforms (no classes, no tiles) on either side of the LDS limits (4224 /
+    10232 points) and long ones through global passes that alternate with the output,
+    first passes staged through LDS: forward out of place, inverse in place."""
+    sizes = [4096, 4225, 5000, 6561, 10000, 10232, 10240, 12288, 19683, 40000, 60060, 65536, 11 * 13 * 256]
+    parts, packed = _packed(sizes, 31)
+    src = vd.asbuffer(packed)
+    dst = vd.Buffer((packed.size,), vd.complex64)
+    plan = R.RaggedFFT(sizes, block=block, classes=0, four_step=False)
+    plan(dst, src)
+    assert not plan.aborted()
+    assert _check(sizes, parts, dst.read(0)) < 1e-5
+    inv = R.RaggedFFT(sizes, inverse=True, block=block, classes=0, four_step=False)
+    inv(dst)
+    assert O.rel_l2(dst.read(0), packed) < 1e-5
 
 
 @pytest.mark.parametrize("four_step,sync", [(False, "launch"), (True, "launch"), (True, "counter")])
