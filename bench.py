@@ -658,10 +658,12 @@ def main():
                 "value": m["value"], "unit": "x", "workload": "3-D C2C 256^3, bf16 storage, fp32 output",
                 "ms_bf16": d["bf16"]["ms"], "ms_fp32": d["fp32"]["ms"],
                 "rel_l2_vs_fp64_quantized": d["bf16"]["rel_l2_vs_fp64_quantized"],
+                "rel_l2_vs_fp64_unquantized": d["bf16"]["rel_l2_vs_fp64_unquantized"],
                 "rel_l2_fp32_path_vs_fp64": d["fp32"]["rel_l2_vs_fp64"],
                 "mfma_utilisation": d["bf16"]["mfma_utilisation"],
                 "fp16_storage_speedup": d["fp16"]["speedup_vs_fp32"],
                 "fp16_rel_l2_vs_fp64_quantized": d["fp16"]["rel_l2_vs_fp64_quantized"],
+                "fp16_rel_l2_vs_fp64_unquantized": d["fp16"]["rel_l2_vs_fp64_unquantized"],
                 "timing": "median of 15 interleaved rounds of back-to-back launches per leg"}
             if "cpu_pocketfft" in m:
                 out["bf16_mfma_vs_fp32"]["cpu_pocketfft"] = m["cpu_pocketfft"]
