@@ -172,7 +172,11 @@ int vkx_chirp(uint64_t dst, int n, int count, int sign, int double_precision, ui
  * waits for items before it; `counters` holds `ncounters` >= 2 device ints, all
  * zeroed by the call (queue position, abort flag, per-transform done counters);
  * `workspace` holds `scratch_per_wg` complex64 elements per workgroup of `grid`;
- * `block` in {64, 128, ..., 512}.  A waiting item polls its counter at most
+ * `block` in {64, 128, ..., 512} (512-thread workgroups take the whole 160 KB of
+ * LDS and keep whole transforms up to 10232 points there; smaller ones 66 KB and
+ * 4224 points; longer whole transforms alternate between the output and their
+ * workspace, which must hold 2 n elements per workgroup for a whole transform of
+ * n > 4096 points).  A waiting item polls its counter at most
  * `spin_limit` times (<= 0: 2^20), then sets the abort flag (counters[1] = 1) and
  * every waiting workgroup stops waiting, so the launch always ends; a result with
  * the flag set is invalid. */
