@@ -76,6 +76,29 @@ def test_plan_lds_and_threads_within_gfx950_limits():
         assert plan.NWG >= 1
 
 
+def test_xcd_tile_order_only_for_rows_off_the_cache_line():
+    """Kernels whose rows or strides start off a 128-byte line, or that carry hooks, take
+    their tiles in XCD-contiguous order (grid a multiple of 8); aligned ones keep the
+    plain order.  The logical order is a permutation of the grid."""
+    f = vd.fft
+    on = lambda k: "XCDMAP = true" in k.source  # noqa: E731
+    assert not on(f.make_fft_shader((1024, 4096), None))            # 32 KB rows
+    assert not on(f.make_fft_shader((256, 256, 256), 1))            # 2 KB stride
+    assert on(f.make_fft_shader((32, 1024, 513), 1))                # 4104-byte stride
+    assert on(f.make_fft_shader((2000, 100), None))                 # 800-byte rows, 80 workgroups
+    assert not on(f.make_fft_shader((64, 100), None))               # 3 workgroups: not a multiple of 8
+    assert on(f.make_convolution_shader((32, 1024, 513), None, 1, 1, kernel_inner_only=True))
+    assert not on(f.make_convolution_shader((64, 512, 256), None, 1, 1, kernel_inner_only=True))
+    from vkdispatch_amd.fft import autograd
+    p = autograd.plans_for(32, 1024, 1024, vd.complex64)
+    assert on(p.rows_fwd) and on(p.rows_inv)                        # 513-value spectrum rows
+    for k in (f.make_fft_shader((32, 1024, 513), 1), f.make_fft_shader((2000, 100), None)):
+        G = k.plan.NWG
+        assert G % 8 == 0 and f"NWG = {G}u" in k.source
+        b = np.arange(G)
+        assert sorted(((b & 7) * (G // 8) + (b >> 3)).tolist()) == list(range(G))
+
+
 def test_pad_dim_and_primes_like_reference():
     from vkdispatch_amd.fft.prime_utils import pad_dim, prime_factors, group_primes
     assert prime_factors(360) == [2, 2, 2, 3, 3, 5]

@@ -299,12 +299,28 @@ struct Geo {
   bool valid;  // this slot maps to a real transform
 };
 
+// Cfg::XCDMAP (optional, with Cfg::NWG a multiple of 8): workgroups are dealt to the
+// XCDs round-robin (blockIdx mod 8), and the logical index makes the workgroups of
+// one XCD take consecutive tiles, so that neighbouring tiles share that XCD's L2 and
+// a cache line straddling two of them (rows gathered from unaligned offsets) leaves
+// memory once.
+template <class C, class = void> struct xcdmap_of { static constexpr bool value = false; };
+template <class C> struct xcdmap_of<C, decltype((void)C::XCDMAP)> { static constexpr bool value = C::XCDMAP; };
+template <class C> __device__ __forceinline__ u32 logical_wg() {
+  if constexpr (xcdmap_of<C>::value) {
+    static_assert(C::NWG % 8u == 0u, "XCD mapping needs a grid of 8k workgroups");
+    return (blockIdx.x & 7u) * (C::NWG / 8u) + (blockIdx.x >> 3);
+  } else {
+    return blockIdx.x;
+  }
+}
+
 template <class C> __device__ __forceinline__ Geo make_geo() {
   Geo g;
   const int tid = (int)threadIdx.x;
   if constexpr (C::COLFAST) { g.b = tid % C::B; g.t = tid / C::B; }
   else { g.t = tid % C::TT; g.b = tid / C::TT; }
-  const u32 wg = blockIdx.x;
+  const u32 wg = logical_wg<C>();
   if constexpr (C::S > 1) {
     g.outer = wg / C::NWG_INNER;
     g.inner = (wg % C::NWG_INNER) * C::B + (u32)g.b;
@@ -712,10 +728,10 @@ template <class C> __device__ __forceinline__ u32 kernel_index_base(const Geo& g
 template <class C> __device__ __forceinline__ u32 transposed_index(const Geo& g, int slot) {
   const u32 tid = threadIdx.x;
   if constexpr (C::KERNEL_INNER_ONLY) {
-    if constexpr (C::S > 1) return ((blockIdx.x % C::NWG_INNER) * C::KSLOTS + (u32)slot) * C::BLOCK + tid;
+    if constexpr (C::S > 1) return ((logical_wg<C>() % C::NWG_INNER) * C::KSLOTS + (u32)slot) * C::BLOCK + tid;
     else return (u32)slot * C::TT + (u32)g.t;
   } else {
-    return (blockIdx.x * C::KSLOTS + (u32)slot) * C::BLOCK + tid;
+    return (logical_wg<C>() * C::KSLOTS + (u32)slot) * C::BLOCK + tid;
   }
 }
 

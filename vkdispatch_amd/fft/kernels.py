@@ -125,7 +125,9 @@ def _pl(passes: Sequence[Tuple[int, int]]) -> str:
 
 def cfg_struct(plan: KernelPlan, *, in_os: int, out_os: int, k_os: int, sig: Tuple[int, int], r2c: bool,
                inverse: bool, normalize: bool, scale: float, knum: int = 1, k_inner_only: bool = False,
-               k_transposed: bool = False) -> str:
+               k_transposed: bool = False, xcd_map: bool = False) -> str:
+    """``struct Cfg`` of a generic FFT-family kernel.  xcd_map: deal the workgroups of one
+    XCD consecutive tiles (applied when the grid is a multiple of 8 workgroups)."""
     b = lambda v: "true" if v else "false"  # noqa: E731
     lines = [
         "struct Cfg {",
@@ -143,8 +145,10 @@ def cfg_struct(plan: KernelPlan, *, in_os: int, out_os: int, k_os: int, sig: Tup
         f"  static constexpr bool KERNEL_INNER_ONLY = {b(k_inner_only)}, KERNEL_TRANSPOSED = {b(k_transposed)};",
         f"  using fwd = {_pl(plan.fwd)};",
         f"  using inv = {_pl(plan.inv)};",
-        "};",
     ]
+    if xcd_map and plan.NWG % 8 == 0 and plan.NWG >= 16:
+        lines.append(f"  static constexpr bool XCDMAP = true; static constexpr vkx::u32 NWG = {plan.NWG}u;")
+    lines.append("};")
     return "\n".join(lines)
 
 

@@ -17,6 +17,12 @@ from .planner import AxisGeometry, KernelPlan, make_plan, transposed_size
 from .precision import kernel_real_type
 
 
+
+def _unaligned(*pitch_bytes: int) -> bool:
+    """Whether rows at these byte pitches start off 128-byte cache lines."""
+    return any(int(p) % 128 for p in pitch_bytes)
+
+
 def _compute_dtype(compute_type):
     return dt.complex128 if compute_type is dt.complex128 else dt.complex64
 
@@ -95,8 +101,12 @@ def make_fft_shader(buffer_shape: Tuple, axis: int = None, inverse: bool = False
         in_os = out_os = N * geom.S
     sig = _resolve_signal_range(input_signal_range, N)
     normalize = bool(inverse and normalize_inverse)
+    # rows at unaligned offsets (hooked gathers and scatters, strides off the cache line)
+    # share lines between neighbouring tiles: those tiles go to one XCD (one fetch)
+    xcd = bool(input_map or output_map) or _unaligned((geom.S if geom.S > 1 else N) * output_type.item_size,
+                                                      (geom.S if geom.S > 1 else N) * (input_type or output_type).item_size)
     cfg = cfg_struct(plan, in_os=in_os, out_os=out_os, k_os=N * geom.S, sig=sig, r2c=r2c, inverse=inverse,
-                     normalize=normalize, scale=1.0 / N)
+                     normalize=normalize, scale=1.0 / N, xcd_map=xcd)
     functors = _io_functors(args, output_map, input_map, output_type, input_type, compute, N, 1)
     kind = ("irfft" if inverse else "rfft") if r2c else "fft"
     tag = ("inv" if inverse else "fwd") + ("_hook" if (input_map or output_map) else "")
@@ -124,8 +134,9 @@ def _make_real_row_shader(geom: AxisGeometry, inverse: bool, normalize_inverse: 
     in_os, out_os = (sp, rp) if inverse else (rp, sp)
     sig = _resolve_signal_range(input_signal_range, N)
     normalize = bool(inverse and normalize_inverse)
+    real_b, spec_b = rp * io_type.child_type.item_size, sp * io_type.item_size
     cfg = cfg_struct(plan, in_os=in_os, out_os=out_os, k_os=0, sig=sig, r2c=True, inverse=inverse,
-                     normalize=normalize, scale=1.0 / N)
+                     normalize=normalize, scale=1.0 / N, xcd_map=_unaligned(real_b, spec_b))
     from .hooks import Arg
     if oop:
         real_t = io_type.child_type
@@ -206,7 +217,9 @@ def make_convolution_shader(buffer_shape: Tuple, kernel_map=None, kernel_num: in
     sig = _resolve_signal_range(input_signal_range, N)
     cfg = cfg_struct(plan, in_os=N * geom.S, out_os=N * geom.S, k_os=N * geom.S, sig=sig, r2c=False,
                      inverse=False, normalize=bool(normalize), scale=1.0 / N, knum=kernel_num,
-                     k_inner_only=kernel_inner_only, k_transposed=transposed_kernel)
+                     k_inner_only=kernel_inner_only, k_transposed=transposed_kernel,
+                     xcd_map=bool(input_map or output_map) or _unaligned(
+                         (geom.S if geom.S > 1 else N) * (output_type or compute_type).item_size))
     functors = _io_functors(args, output_map, input_map, output_type, input_type, compute, N, kernel_num)
     if kernel_map is not None:
         functors.append(hooks.functor_source("HookKernel", kernel_map, k_args, compute, N, writing=False,
