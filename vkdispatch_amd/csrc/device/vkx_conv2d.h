@@ -411,9 +411,15 @@ __device__ __forceinline__ void conv2d_phase_body(const KParams& prm) {
       conv_kprep<C>(a.K, a.kt, wg - C::NPAIRS * SH::NA, tid);
     }
   } else if constexpr (PHASE == 1) {
-    if (wg < NPG * NBT) {
-      if constexpr (CM) conv_cols_cm<C>(a, a.kt, lds, (wg % NPG) * C::PP, wg / NPG, tid);
-      else conv_cols<C>(a, a.kt, lds, wg % C::NPAIRS, wg / C::NPAIRS, tid, wave, lane);
+    // Task t = (column slice t / NPG, pair group t % NPG).  Workgroups are dealt to the
+    // 8 XCDs round-robin; on a grid of exactly the 8k tasks, the workgroups of one XCD
+    // take consecutive tasks, so each slice of Kt is read by one XCD (from its L2 for
+    // the slice's other pair groups) instead of by all eight.
+    constexpr u32 T = NPG * NBT;
+    const u32 t = (T % 8u == 0u && gridDim.x == T) ? (wg & 7u) * (T / 8u) + (wg >> 3) : wg;
+    if (t < T) {
+      if constexpr (CM) conv_cols_cm<C>(a, a.kt, lds, (t % NPG) * C::PP, t / NPG, tid);
+      else conv_cols<C>(a, a.kt, lds, t % C::NPAIRS, t / C::NPAIRS, tid, wave, lane);
     }
   } else {
     if (wg < C::NPAIRS * SH::NA) {
