@@ -134,9 +134,10 @@ def _make_real_row_shader(geom: AxisGeometry, inverse: bool, normalize_inverse: 
     in_os, out_os = (sp, rp) if inverse else (rp, sp)
     sig = _resolve_signal_range(input_signal_range, N)
     normalize = bool(inverse and normalize_inverse)
-    real_b, spec_b = rp * io_type.child_type.item_size, sp * io_type.item_size
+    # plain tile order: with the XCD-contiguous order the 1024-point rows of 513-value
+    # spectra measured slower (67.1 / 57.2 against 61.6 / 54.2 us)
     cfg = cfg_struct(plan, in_os=in_os, out_os=out_os, k_os=0, sig=sig, r2c=True, inverse=inverse,
-                     normalize=normalize, scale=1.0 / N, xcd_map=_unaligned(real_b, spec_b))
+                     normalize=normalize, scale=1.0 / N)
     from .hooks import Arg
     if oop:
         real_t = io_type.child_type
