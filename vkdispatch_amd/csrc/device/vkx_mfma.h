@@ -645,9 +645,16 @@ __device__ __forceinline__ void fft3_p2_body(const KParams& prm) {
   const cpx<_Float16>* mid = bufp<const cpx<_Float16>>(prm, 1);
   cpx<float>* out = bufp<cpx<float>>(prm, 0);
   const cpx<float>* roots = bufp<const cpx<float>>(prm, SLOT_ROOTS);
-  // consecutive workgroups of one XCD take consecutive k2 blocks of one c1
+  // The eight XCDs work on the same c1 at a time, so the output rows of one c1 are
+  // written by all of them together: XCD x takes the groups of four consecutive k2
+  // blocks q with q mod 8 = x, the four blocks of a group (which complete each other's
+  // 128-byte output lines) on consecutive workgroups of that XCD.  (c1 per XCD and
+  // consecutive k2 blocks of one c1 in turn: 0.9 us slower per transform.)
+  static_assert(KB % 32 == 0, "k2 blocks per c1: groups of four on each of the eight XCDs");
   const u32 b = blockIdx.x, xcd = b & 7, slot = b >> 3;
-  const u32 kb = slot % KB, c1 = (slot / KB) * 8 + xcd;
+  constexpr u32 PER = KB / 8;  // blocks of one c1 per XCD
+  const u32 c1 = slot / PER, loc = slot % PER;
+  const u32 kb = ((loc / 4) * 8 + xcd) * 4 + loc % 4;
   constexpr bool VH = C::VH;  // tile results held as fp16 pairs, scaled by 1/16
   constexpr float VS = VH ? 1.f / 16.f : 1.f;
   Dft16Ops ops = dft16_ops<C::DIR>(roots, r, g);
